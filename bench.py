@@ -1,0 +1,177 @@
+"""Benchmark of the mEVP dynamics step on one MI355X.
+
+A step is one model timestep of the benchmark box as MEVPDynamics::update runs it
+(MEVPDynamics.cpp:59-87): DG0 reset of h and A, the moving-cyclone wind evaluated on
+the device, then VPCGDynamicsKernel::update (advection of A and h with limiters,
+prepareIteration, 100 mEVP subcycles). Inputs are resident in HBM before timing.
+
+Metric: Mcell-mEVP-iters/s = elements x subcycles / wall time / 1e6, FP64 path, on
+the 512 x 512 CG2/DG2 box (dx = 1 km, dt = 120 s). Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def byte_model(nx, ny, cg=2, dg=6, we=8):
+    """Algorithmic HBM bytes of one subcycle (element pass, node pass)."""
+    S = 8 if cg == 2 else 3
+    ne = nx * ny
+    nn = (cg * nx + 1) * (cg * ny + 1)
+    elem = we * (6 * S + 2 * dg) * ne + 8 * 2 * nn   # sigma r+w, h, A; u, v at nodes
+    node = we * 3 * S * ne + 8 * 15 * nn            # sigma r; 13 node fields r, u, v w
+    return elem, node
+
+
+def make_kernel(nx, ny, seed, prec):
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    s = inputs.benchmark_setup(nx, ny, seed)
+    k = nd.MEVPDynamicsKernel(2, 6)
+    k.initialise(s["coords"], False, s["mask"], nx, ny)
+    k.setData("hice", s["h"], 1)
+    k.setData("cice", s["A"], 1)
+    k.setData("uocean", s["uocean"], 1)
+    k.setData("vocean", s["vocean"], 1)
+    k.setData("ssh", 0 * s["h"], 1)
+    if prec:
+        k.set_precision(prec)
+    return k, inputs.cyclone_offset(seed), inputs.L_BOX
+
+
+def run_steps(k, off, L, t0, n, dt):
+    for i in range(n):
+        k.reset_dg0("hice")
+        k.reset_dg0("cice")
+        k.benchmark_wind(L, (t0 + i) * dt, off[0], off[1], 30.0)
+        k.update(dt)
+
+
+def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0):
+    k, off, L = make_kernel(nx, ny, seed, prec)
+    run_steps(k, off, L, 0, warmup, dt)
+    k.synchronize()
+    t = time.perf_counter()
+    run_steps(k, off, L, warmup, steps, dt)
+    k.synchronize()
+    el = time.perf_counter() - t
+    # per-kernel durations with HIP events on the library's stream
+    k.set_profiling(True)
+    run_steps(k, off, L, warmup + steps, 1, dt)
+    k.synchronize()
+    stats = {n: k.kernel_stats(n) for n in ("mevp_element", "mevp_node", "transport_op", "dg2cg", "cg2dg")}
+    k.set_profiling(False)
+    return el, stats
+
+
+def cpu_baseline(nx, ny, seed=0, target_s=12.0):
+    """The FP64 oracle (OpenMP C++ restatement of the reference) on the host cores:
+    mEVP subcycles of the same box, timed until about target_s seconds."""
+    from nextsimdg_amd import inputs
+    from oracle import Oracle, OracleDyn
+
+    s = inputs.benchmark_setup(nx, ny, seed)
+    o = OracleDyn(2, 6)
+    o.initialise(nx, ny, s["coords"], s["mask"])
+    uw, vw = Oracle().benchmark_wind(nx, ny, inputs.L_BOX, 0.0, *inputs.cyclone_offset(seed), 30.0)
+    for name, v in (("hice", s["h"]), ("cice", s["A"]), ("uocean", s["uocean"]), ("vocean", s["vocean"]),
+                    ("uwind", uw), ("vwind", vw), ("ssh", 0 * uw)):
+        o.set_field(name, v, 1)
+    o.advection(120.0)
+    o.prepare_iteration()
+    o.begin_iteration(120.0)
+    o.run_pass("subcycle", 1)  # warm
+    n, t = 0, time.perf_counter()
+    while time.perf_counter() - t < target_s:
+        o.run_pass("subcycle", 1)
+        n += 1
+    el = time.perf_counter() - t
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(nx * ny * n / el / 1e6, 4),
+        "unit": "Mcell-mEVP-iters/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} mEVP subcycles of the {nx}x{ny} CG2/DG2 box on the FP64 oracle "
+                  f"(advection excluded), {el:.1f} s, OMP_NUM_THREADS={threads}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design "
+              "note in DESIGN.md, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    nx = ny = args.nx
+    log(f"bench: {nx}x{ny} CG2/DG2 FP64, warmup {args.warmup}, steps {args.steps}")
+    el, stats = measure(nx, ny, args.steps, args.warmup)
+    iters = nx * ny * 100 * args.steps
+    value = iters / el / 1e6
+    eb, nb = byte_model(nx, ny)
+    ms = {n: (c, t / c if c else None) for n, (c, t) in stats.items()}
+    dom = max(("mevp_element", "mevp_node"), key=lambda n: stats[n][1])
+    avg_ms = ms[dom][1]
+    alg = eb if dom == "mevp_element" else nb
+    achieved = alg / (avg_ms * 1e-3)
+    traffic = None
+    if os.path.exists(args.traffic):
+        try:
+            traffic = json.load(open(args.traffic)).get(f"{dom}_{nx}")
+        except (ValueError, OSError):
+            traffic = None
+    out = {
+        "metric": "Mcell-mEVP-iters/s",
+        "value": round(value, 3),
+        "unit": "Mcell-mEVP-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"mEVP dynamics step, benchmark box {nx}x{ny}, CG2 velocity, DG2 advection, "
+                               f"8-dof DG stress, FP64, dt=120 s, 100 subcycles, moving cyclone",
+                   "nx": nx, "ny": ny, "subcycles": 100, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 5)},
+        "kernels_ms": {n: (round(v[1], 5) if v[1] else None) for n, v in ms.items()},
+        "subcycle_hbm_frac": round((eb + nb) / ((stats["mevp_element"][1] + stats["mevp_node"][1])
+                                                / stats["mevp_element"][0] * 1e-3) / HBM_PEAK, 4),
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle) ...")
+        try:
+            out["cpu_baseline"] = cpu_baseline(nx, ny)
+        except Exception as e:  # noqa: BLE001 - the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,426 @@
+// extern "C" entry points of include/nsdg_dyn.h. Every call is guarded: C++
+// exceptions become a negative status and a per-thread message.
+#include "nsdg_dyn.hpp"
+#include "nsdg_transport.hpp"
+
+#include <cstring>
+
+using namespace nsdg;
+
+struct nsdg_mesh {
+    HostMesh m;
+};
+struct nsdg_transport {
+    TransportHandle h;
+};
+struct nsdg_dyn {
+    std::unique_ptr<DynBase> k;
+    int cg, dg;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F> int guard(F&& f)
+{
+    try {
+        f();
+        return NSDG_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return NSDG_ERR_ARG;
+    } catch (...) {
+        g_err = "unknown error";
+        return NSDG_ERR_ARG;
+    }
+}
+void needDevice()
+{
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1)
+        throw Error(NSDG_ERR_HIP, "no HIP device available");
+}
+template <class T> T* nonNull(T* p, const char* what)
+{
+    if (!p)
+        throw Error(NSDG_ERR_ARG, std::string(what) + " is NULL");
+    return p;
+}
+} // namespace
+
+extern "C" {
+
+const char* nsdg_last_error(void) { return g_err.c_str(); }
+
+int nsdg_params_default(nsdg_params* p)
+{
+    return guard([&] {
+        nonNull(p, "params");
+        p->rho_ice = 900.0;
+        p->rho_atm = 1.3;
+        p->rho_ocean = 1026.0;
+        p->C_atm = 1.2e-3;
+        p->C_ocean = 5.5e-3;
+        p->F_atm = p->C_atm * p->rho_atm;
+        p->F_ocean = p->C_ocean * p->rho_ocean;
+        p->fc = 1.45842e-4;
+        p->ocean_turning_angle = 0.0;
+        p->gravity = 9.81;
+        p->Pstar = 27500.0;
+        p->DeltaMin = 2.e-9;
+        p->alpha = 1500.0;
+        p->beta = 1500.0;
+        p->nsteps = 100;
+    });
+}
+
+int nsdg_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return NSDG_ERR_HIP;
+    return n;
+}
+
+// ---------------------------------------------------------------- mesh
+int nsdg_mesh_create(size_t nx, size_t ny, const double* coords, int spherical, nsdg_mesh** out)
+{
+    return guard([&] {
+        nonNull(out, "out");
+        require(!spherical, "spherical coordinates are not supported", NSDG_ERR_UNSUPPORTED);
+        auto m = std::make_unique<nsdg_mesh>();
+        m->m.create(nx, ny, coords);
+        *out = m.release();
+    });
+}
+int nsdg_mesh_set_landmask(nsdg_mesh* m, const double* mask)
+{
+    return guard([&] { nonNull(m, "mesh")->m.setLandmask(mask); });
+}
+int nsdg_mesh_dirichlet_from_mask(nsdg_mesh* m)
+{
+    return guard([&] { nonNull(m, "mesh")->m.dirichletFromMask(); });
+}
+int nsdg_mesh_dirichlet_from_edge(nsdg_mesh* m, int edge)
+{
+    return guard([&] { nonNull(m, "mesh")->m.dirichletFromEdge(edge); });
+}
+int nsdg_mesh_set_dirichlet(nsdg_mesh* m, int edge, const uint64_t* ids, size_t n)
+{
+    return guard([&] {
+        nonNull(m, "mesh");
+        require(edge >= 0 && edge < 4, "edge must be 0..3");
+        require(ids != nullptr || n == 0, "ids is NULL");
+        for (size_t i = 0; i < n; ++i)
+            require(ids[i] < m->m.ne(), "Dirichlet element id out of range");
+        m->m.dirichlet[edge].assign(ids, ids + n);
+    });
+}
+int nsdg_mesh_add_periodic(nsdg_mesh* m, const uint64_t* entries, size_t n)
+{
+    return guard([&] {
+        nonNull(m, "mesh");
+        require(entries != nullptr || n == 0, "entries is NULL");
+        m->m.addPeriodic(entries, n);
+        m->m.validate();
+    });
+}
+int nsdg_mesh_get_dirichlet(const nsdg_mesh* m, int edge, uint64_t* out, size_t* n)
+{
+    return guard([&] {
+        nonNull(m, "mesh");
+        nonNull(n, "n");
+        require(edge >= 0 && edge < 4, "edge must be 0..3");
+        const auto& d = m->m.dirichlet[edge];
+        if (out)
+            std::memcpy(out, d.data(), d.size() * sizeof(uint64_t));
+        *n = d.size();
+    });
+}
+int nsdg_mesh_destroy(nsdg_mesh* m)
+{
+    delete m;
+    return NSDG_OK;
+}
+
+// ---------------------------------------------------------------- transport
+int nsdg_transport_create(const nsdg_mesh* m, int dg, nsdg_transport** out)
+{
+    return guard([&] {
+        nonNull(m, "mesh");
+        nonNull(out, "out");
+        needDevice();
+        auto t = std::make_unique<nsdg_transport>();
+        NSDG_HIP(hipStreamCreateWithFlags(&t->h.stream, hipStreamNonBlocking));
+        t->h.mesh.build(m->m, t->h.stream);
+        t->h.t = makeTransport(dg, t->h.mesh, t->h.stream, t->h.prof);
+        t->h.field.alloc(t->h.mesh.ne * dg);
+        NSDG_HIP(hipStreamSynchronize(t->h.stream));
+        *out = t.release();
+    });
+}
+int nsdg_transport_set_scheme(nsdg_transport* t, const char* scheme)
+{
+    return guard([&] {
+        nonNull(t, "transport");
+        const std::string s(nonNull(scheme, "scheme"));
+        require(s == "rk1" || s == "rk2" || s == "rk3", "scheme must be rk1, rk2 or rk3");
+        t->h.t->setScheme(s[2] - '0');
+    });
+}
+int nsdg_transport_set_velocity(nsdg_transport* t, const double* vx, const double* vy)
+{
+    return guard([&] { nonNull(t, "transport")->h.t->setVelocityHost(nonNull(vx, "vx"), nonNull(vy, "vy")); });
+}
+int nsdg_transport_prepare_advection(nsdg_transport* t, int cg, const double* u, const double* v)
+{
+    return guard([&] { nonNull(t, "transport")->h.t->prepareAdvectionHost(cg, nonNull(u, "u"), nonNull(v, "v")); });
+}
+int nsdg_transport_reinit_normal_velocity(nsdg_transport* t)
+{
+    return guard([&] {
+        nonNull(t, "transport")->h.t->reinitNormalVelocity();
+        NSDG_HIP(hipStreamSynchronize(t->h.stream));
+    });
+}
+int nsdg_transport_run(nsdg_transport* t, double dt, double* phi, int nsteps, int limit_flags)
+{
+    return guard([&] {
+        nonNull(t, "transport");
+        nonNull(phi, "phi");
+        require(nsteps >= 0, "nsteps must be >= 0");
+        auto& h = t->h;
+        const int dg = h.t->dg();
+        DBuf<double> scratch;
+        uploadSoA(scratch, phi, h.mesh.ne, dg, h.field.p, h.stream);
+        for (int k = 0; k < nsteps; ++k) {
+            h.t->reinitNormalVelocity();
+            h.t->step(dt, h.field.p);
+            if (limit_flags & 3)
+                h.t->limit(h.field.p, (limit_flags & 1) != 0, 1.0, (limit_flags & 2) != 0, 0.0);
+        }
+        downloadSoA(scratch, h.field.p, h.mesh.ne, dg, phi, h.stream);
+    });
+}
+int nsdg_transport_step(nsdg_transport* t, double dt, double* phi)
+{
+    return guard([&] {
+        nonNull(t, "transport");
+        nonNull(phi, "phi");
+        auto& h = t->h;
+        const int dg = h.t->dg();
+        DBuf<double> scratch;
+        uploadSoA(scratch, phi, h.mesh.ne, dg, h.field.p, h.stream);
+        h.t->step(dt, h.field.p);
+        downloadSoA(scratch, h.field.p, h.mesh.ne, dg, phi, h.stream);
+    });
+}
+int nsdg_transport_destroy(nsdg_transport* t)
+{
+    delete t;
+    return NSDG_OK;
+}
+
+int nsdg_limit(int dg, size_t n, double* phi, double bound, int is_max)
+{
+    return guard([&] {
+        nonNull(phi, "phi");
+        require(dg == 1 || dg == 3 || dg == 6, "limiters are defined for dg 1, 3, 6");
+        require(n >= 1, "n must be >= 1");
+        needDevice();
+        TransportHandle h;
+        NSDG_HIP(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+        // A strip mesh of at least 2 x 2 elements carries the field in its first n elements.
+        const size_t nxm = n < 2 ? 2 : n;
+        HostMesh m;
+        std::vector<double> coords(2 * (nxm + 1) * 3, 0.0);
+        for (size_t j = 0; j < 3; ++j)
+            for (size_t i = 0; i <= nxm; ++i) {
+                coords[2 * (i + (nxm + 1) * j)] = static_cast<double>(i);
+                coords[2 * (i + (nxm + 1) * j) + 1] = static_cast<double>(j);
+            }
+        m.create(nxm, 2, coords.data());
+        h.mesh.build(m, h.stream);
+        h.t = makeTransport(dg, h.mesh, h.stream, h.prof);
+        DBuf<double> f, scratch;
+        f.alloc(h.mesh.ne * dg);
+        deviceFill(f.p, f.n, 0.0, h.stream);
+        std::vector<double> full(h.mesh.ne * dg, 0.0);
+        std::memcpy(full.data(), phi, n * dg * sizeof(double));
+        uploadSoA(scratch, full.data(), h.mesh.ne, dg, f.p, h.stream);
+        h.t->limit(f.p, is_max != 0, bound, is_max == 0, bound);
+        downloadSoA(scratch, f.p, h.mesh.ne, dg, full.data(), h.stream);
+        std::memcpy(phi, full.data(), n * dg * sizeof(double));
+    });
+}
+
+// ---------------------------------------------------------------- dynamics
+int nsdg_dyn_create(int cg_degree, int dg_adv, const nsdg_params* params, nsdg_dyn** out)
+{
+    return guard([&] {
+        nonNull(out, "out");
+        needDevice();
+        nsdg_params p;
+        if (params)
+            p = *params;
+        else
+            nsdg_params_default(&p);
+        require(p.alpha > 0 && p.beta > 0 && p.nsteps >= 0, "alpha, beta must be > 0 and nsteps >= 0");
+        auto d = std::make_unique<nsdg_dyn>();
+        d->k = makeDyn(cg_degree, dg_adv, p);
+        d->cg = cg_degree;
+        d->dg = dg_adv;
+        *out = d.release();
+    });
+}
+int nsdg_dyn_initialise(nsdg_dyn* d, size_t nx, size_t ny, const double* coords, int spherical, const double* mask)
+{
+    return guard([&] {
+        nonNull(d, "dyn");
+        require(!spherical, "spherical coordinates are not supported", NSDG_ERR_UNSUPPORTED);
+        HostMesh m;
+        m.create(nx, ny, coords);
+        m.setLandmask(mask);
+        m.dirichletFromMask();
+        for (int e = 0; e < 4; ++e)
+            m.dirichletFromEdge(e);
+        d->k->initialise(m);
+    });
+}
+int nsdg_dyn_initialise_mesh(nsdg_dyn* d, const nsdg_mesh* m)
+{
+    return guard([&] { nonNull(d, "dyn")->k->initialise(nonNull(m, "mesh")->m); });
+}
+int nsdg_dyn_set_field(nsdg_dyn* d, const char* name, const double* data, int ncomp)
+{
+    return guard([&] { nonNull(d, "dyn")->k->setField(nonNull(name, "name"), data, ncomp); });
+}
+int nsdg_dyn_get_field(nsdg_dyn* d, const char* name, double* out, int ncomp)
+{
+    return guard([&] { nonNull(d, "dyn")->k->getField(nonNull(name, "name"), out, ncomp); });
+}
+int nsdg_dyn_get_cg(nsdg_dyn* d, const char* name, double* out)
+{
+    return guard([&] { nonNull(d, "dyn")->k->getCG(nonNull(name, "name"), nonNull(out, "out")); });
+}
+int nsdg_dyn_set_cg(nsdg_dyn* d, const char* name, const double* in)
+{
+    return guard([&] { nonNull(d, "dyn")->k->setCG(nonNull(name, "name"), nonNull(in, "in")); });
+}
+size_t nsdg_dyn_cg_nodes(const nsdg_dyn* d) { return d ? d->k->cgNodes() : 0; }
+int nsdg_dyn_update(nsdg_dyn* d, double dt)
+{
+    return guard([&] { nonNull(d, "dyn")->k->update(dt); });
+}
+int nsdg_dyn_advect(nsdg_dyn* d, double dt)
+{
+    return guard([&] { nonNull(d, "dyn")->k->advect(dt); });
+}
+int nsdg_dyn_prepare_iteration(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->prepareIteration(); });
+}
+int nsdg_dyn_begin_iteration(nsdg_dyn* d, double dt)
+{
+    return guard([&] { nonNull(d, "dyn")->k->beginIteration(dt); });
+}
+int nsdg_dyn_strain(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->strain(); });
+}
+int nsdg_dyn_stress_update(nsdg_dyn* d, int repeat)
+{
+    return guard([&] {
+        require(repeat >= 0, "repeat must be >= 0");
+        nonNull(d, "dyn")->k->stress(repeat);
+    });
+}
+int nsdg_dyn_divergence(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->divergence(); });
+}
+int nsdg_dyn_momentum(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->momentum(); });
+}
+int nsdg_dyn_apply_boundaries(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->boundaries(); });
+}
+int nsdg_dyn_subcycles(nsdg_dyn* d, int n)
+{
+    return guard([&] {
+        require(n >= 0, "n must be >= 0");
+        nonNull(d, "dyn")->k->subcycles(n);
+    });
+}
+int nsdg_dyn_set_precision(nsdg_dyn* d, int precision)
+{
+    return guard([&] { nonNull(d, "dyn")->k->setPrecision(precision); });
+}
+int nsdg_dyn_benchmark_wind(nsdg_dyn* d, double L, double t, double xoff, double yoff, double vmax)
+{
+    return guard([&] { nonNull(d, "dyn")->k->benchmarkWind(L, t, xoff, yoff, vmax); });
+}
+int nsdg_dyn_reset_dg0(nsdg_dyn* d, const char* name)
+{
+    return guard([&] { nonNull(d, "dyn")->k->resetDG0(nonNull(name, "name")); });
+}
+int nsdg_dyn_mass(nsdg_dyn* d, double* vol, double* area)
+{
+    return guard([&] { nonNull(d, "dyn")->k->mass(vol, area); });
+}
+int nsdg_dyn_synchronize(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->synchronize(); });
+}
+int nsdg_dyn_save_state(nsdg_dyn* d, void* buf, size_t* size)
+{
+    return guard([&] {
+        nonNull(d, "dyn");
+        nonNull(size, "size");
+        const std::vector<char> blob = d->k->saveState();
+        if (buf) {
+            require(*size >= blob.size(), "buffer too small for the state blob");
+            std::memcpy(buf, blob.data(), blob.size());
+        }
+        *size = blob.size();
+    });
+}
+int nsdg_dyn_load_state(nsdg_dyn* d, const void* buf, size_t size)
+{
+    return guard([&] { nonNull(d, "dyn")->k->loadState(static_cast<const char*>(nonNull(buf, "buf")), size); });
+}
+int nsdg_dyn_set_profiling(nsdg_dyn* d, int on)
+{
+    return guard([&] {
+        auto& p = nonNull(d, "dyn")->k->profiler();
+        d->k->synchronize();
+        p.on = on != 0;
+        if (p.on)
+            p.stats.clear();
+    });
+}
+int nsdg_dyn_kernel_stats(nsdg_dyn* d, const char* kernel, int* count, double* total_ms)
+{
+    return guard([&] {
+        nonNull(d, "dyn")->k->synchronize();
+        auto& st = d->k->profiler().stats;
+        auto it = st.find(nonNull(kernel, "kernel"));
+        if (count)
+            *count = it == st.end() ? 0 : it->second.first;
+        if (total_ms)
+            *total_ms = it == st.end() ? 0.0 : it->second.second;
+    });
+}
+int nsdg_dyn_destroy(nsdg_dyn* d)
+{
+    delete d;
+    return NSDG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,1343 @@
+// mEVP momentum/rheology subcycle and the per-timestep driver on gfx950.
+//
+// One mEVP subcycle is two bulk-synchronous HBM passes:
+//  * element pass: strain rate from the 9 (CG2) / 4 (CG1) velocity nodes, the mEVP
+//    stress relaxation at the Gauss points and its projection back to the DG stress
+//    basis, in registers; reads u, v, h, A, sigma, writes sigma. Strain never
+//    touches HBM.
+//  * node pass: a 256-thread block takes a tile of elements plus a one-element halo,
+//    computes each element's weak-divergence contributions to its CG nodes into LDS,
+//    then every CG node of the tile sums its <= 4 contributions in the reference's
+//    two-colour stripe order (atomic-free, deterministic) and applies the FP64
+//    implicit velocity update and the Dirichlet zero.
+// On a uniform mesh one copy of each element operator is read with wave-uniform
+// addresses (scalar loads through the constant cache); otherwise operators are
+// streamed per element, k-major. Per-step set-up (CG2DG, normal velocities, the two
+// transported fields, limiters, DG2CG, sea-surface-height gradient) are further
+// element- or node-parallel kernels. Nothing runs on the host between launches.
+//
+// Reference: dynamics/src/include/VPCGDynamicsKernel.hpp:63-172,
+// dynamics/src/CGDynamicsKernel.cpp:23-444, MEVPStressUpdateStep.hpp:30-118,
+// ParametricMap.cpp:94-356, Interpolations.cpp:129-204, DynamicsKernel.hpp:93-172.
+#include "nsdg_device.hpp"
+#include "nsdg_dyn.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace nsdg {
+
+// ---------------------------------------------------------------- momentum operators
+// Layout (k-major): iMgradX[s*N+j], iMgradY, iMJwPSI[s*Q+q], divS1[j*S+s], divS2,
+// dX_SSH[j*4+k], dY_SSH (ParametricMap.hpp:62-131).
+template <int CG> struct MomLayout {
+    static constexpr int S = cg2dgstress(CG), G = gp1d(S), Q = G * G, N = cgdofs(CG);
+    static constexpr int GX = 0, GY = S * N, B = 2 * S * N, D1 = 2 * S * N + S * Q, D2 = 3 * S * N + S * Q,
+                         SX = 4 * S * N + S * Q, SY = SX + 16, K = SY + 16;
+};
+
+// InitializeDivSMatrices (ParametricMap.cpp:209-297), Cartesian.
+template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* ops)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, G = L::G, Q = L::Q, N = L::N;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nOps)
+        return;
+    auto put = [&](int k, double v) { ops[static_cast<size_t>(k) * nOps + e] = v; };
+    double c[4][2], Fx[2][Q], Fy[2][Q], J[Q];
+    elementCoords(m, e, c);
+    mapGradients<G>(c, Fx, Fy);
+    jacobian<G>(c, J);
+    for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < Q; ++q) {
+            Fx[r][q] *= W<G>::v[0][q];
+            Fy[r][q] *= W<G>::v[0][q];
+        }
+    // mapped, weighted CG gradients: dx_cg = PHIx Fy1 - PHIy Fx1, dy_cg = PHIy Fx0 - PHIx Fy0
+    double d1[N][S], d2[N][S];
+    for (int j = 0; j < N; ++j)
+        for (int s = 0; s < S; ++s) {
+            double a = 0, b = 0;
+            for (int q = 0; q < Q; ++q) {
+                const double dxc = Phi<CG, G>::x[j][q] * Fy[1][q] - Phi<CG, G>::y[j][q] * Fx[1][q];
+                const double dyc = Phi<CG, G>::y[j][q] * Fx[0][q] - Phi<CG, G>::x[j][q] * Fy[0][q];
+                a += dxc * Psi<S, G>::v[s][q];
+                b += dyc * Psi<S, G>::v[s][q];
+            }
+            d1[j][s] = a;
+            d2[j][s] = b;
+            put(L::D1 + j * S + s, a);
+            put(L::D2 + j * S + s, b);
+        }
+    for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 4; ++k) {
+            double a = 0, b = 0;
+            for (int q = 0; q < Q; ++q) {
+                const double dxc = Phi<1, G>::x[j][q] * Fy[1][q] - Phi<1, G>::y[j][q] * Fx[1][q];
+                const double dyc = Phi<1, G>::y[j][q] * Fx[0][q] - Phi<1, G>::x[j][q] * Fy[0][q];
+                a += dxc * Phi<1, G>::v[k][q];
+                b += dyc * Phi<1, G>::v[k][q];
+            }
+            put(L::SX + j * 4 + k, a);
+            put(L::SY + j * 4 + k, b);
+        }
+    double M[S][S], Mi[S][S];
+    massMatrix<S>(c, M);
+    invert<S>(M, Mi);
+    for (int s = 0; s < S; ++s) {
+        for (int j = 0; j < N; ++j) {
+            double a = 0, b = 0;
+            for (int t = 0; t < S; ++t) {
+                a += Mi[s][t] * d1[j][t];
+                b += Mi[s][t] * d2[j][t];
+            }
+            put(L::GX + s * N + j, a);
+            put(L::GY + s * N + j, b);
+        }
+        for (int q = 0; q < Q; ++q) {
+            double a = 0;
+            for (int t = 0; t < S; ++t)
+                a += Mi[s][t] * (Psi<S, G>::v[t][q] * (W<G>::v[0][q] * J[q]));
+            put(L::B + s * Q + q, a);
+        }
+    }
+}
+
+// InitializeLumpedCGMassMatrix (ParametricMap.cpp:94-206): node-centric gather of the
+// element integrals of the CG basis (even element rows first, as the stripes add them).
+template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
+{
+    constexpr int N = cgdofs(CG);
+    const int row = CG * m.nx + 1, nn = row * (CG * m.ny + 1);
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    const Adj adj = adjacent<CG>(n % row, n / row, m.nx, m.ny, true);
+    double acc = 0;
+    for (int k = 0; k < adj.n; ++k) {
+        double c[4][2], J[GG * GG];
+        elementCoords(m, adj.e[k], c);
+        jacobian<GG>(c, J);
+        double me = 0;
+        for (int q = 0; q < GG * GG; ++q)
+            me += Phi<CG, GG>::v[adj.loc[k]][q] * (J[q] * W<GG>::v[0][q]);
+        acc += me;
+    }
+    (void)N;
+    out[n] = acc;
+}
+
+// ---------------------------------------------------------------- DG -> CG
+// Interpolations::DG2CG (Interpolations.cpp:129-204): Lagrange-point values weighted
+// 1/4 (corner), 1/2 (edge midpoint), 1 (centre), gathered odd element rows first,
+// doubled on each domain boundary the node lies on. clamp: 0 none, 1 H (>= 1e-4),
+// 2 A (<= 1 then >= 1e-4), the limits of prepareIteration (CGDynamicsKernel.cpp:272-275).
+template <int CG, int D>
+__global__ __launch_bounds__(kBlock) void k_dg2cg(int nx, int ny, int ne, const double* s1, double* d1, int clamp1,
+    const double* s2, double* d2, int clamp2)
+{
+    const int row = CG * nx + 1, nn = row * (CG * ny + 1);
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    const int nix = n % row, niy = n / row;
+    const Adj adj = adjacent<CG>(nix, niy, nx, ny, false);
+    double a1 = 0, a2 = 0;
+    for (int k = 0; k < adj.n; ++k) {
+        const int e = adj.e[k], j = adj.loc[k];
+        const int jx = j % (CG + 1), jy = j / (CG + 1);
+        const double w = CG == 1 ? 0.25 : ((jx == 1 && jy == 1) ? 1.0 : ((jx == 1 || jy == 1) ? 0.5 : 0.25));
+        double t1 = 0, t2 = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const double l = PsiL<D, CG + 1>::v[i][j];
+            t1 += s1[static_cast<size_t>(i) * ne + e] * l;
+            if (s2)
+                t2 += s2[static_cast<size_t>(i) * ne + e] * l;
+        }
+        a1 += w * t1;
+        a2 += w * t2;
+    }
+    double f = 1.0;
+    if (niy == 0 || niy == CG * ny)
+        f *= 2.0;
+    if (nix == 0 || nix == CG * nx)
+        f *= 2.0;
+    auto clampv = [](double x, int c) {
+        if (c == 1)
+            return fmax(x, 1.e-4);
+        if (c == 2)
+            return fmax(fmin(x, 1.0), 1.e-4);
+        return x;
+    };
+    d1[n] = clampv(a1 * f, clamp1);
+    if (s2)
+        d2[n] = clampv(a2 * f, clamp2);
+}
+template <int CG> __global__ void k_clamp_HA(double* H, double* A, int nn)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    A[n] = fmax(fmin(A[n], 1.0), 1.e-4);
+    H[n] = fmax(H[n], 1.e-4);
+}
+
+// CGAveragePeriodic (VectorManipulations.hpp:26-65): sequential, as in the reference.
+template <int CG> __global__ void k_periodic_average(int nx, const int32_t* ent, int n, double* v1, double* v2)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    const int row = CG * nx + 1;
+    for (int k = 0; k < n; ++k) {
+        const int type = ent[4 * k], rt = ent[4 * k + 1], lb = ent[4 * k + 2];
+        const int i0lb = row * CG * (lb / nx) + CG * (lb % nx);
+        const int i0rt = row * CG * (rt / nx) + CG * (rt % nx);
+        for (int f = 0; f < 2; ++f) {
+            double* v = f == 0 ? v1 : v2;
+            if (!v)
+                continue;
+            for (int j = 0; j <= CG; ++j) {
+                const int i1 = type == 0 ? i0lb + j : i0lb + j * row;
+                const int i2 = type == 0 ? i0rt + CG * row + j : i0rt + CG + j * row;
+                v[i1] = 0.5 * (v[i1] + v[i2]);
+                v[i2] = v[i1];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sea-surface height
+// ComputeGradientOfSeaSurfaceHeight (CGDynamicsKernel.cpp:125-258) in three passes:
+// CG1 nodal SSH (k_dg2cg<1,1>), the CG1 weak gradient divided by the CG1 lumped mass
+// (this kernel), then the boundary extension fused into the CG2 prolongation.
+template <bool UNI>
+__global__ void k_ssh_gradient(int nx, int ny, const double* cgS, const double* ops, size_t stride, int sx, int sy,
+    const double* lumped1, double* gu, double* gv)
+{
+    const int row = nx + 1, nn = row * (ny + 1);
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    const Ops<UNI> op { ops, stride };
+    const Adj adj = adjacent<1>(n % row, n / row, nx, ny, true);
+    double u = 0, v = 0;
+    for (int k = 0; k < adj.n; ++k) {
+        const int e = adj.e[k], j = adj.loc[k];
+        const int id = (e / nx) * row + e % nx;
+        const double l[4] = { cgS[id], cgS[id + 1], cgS[id + row], cgS[id + row + 1] };
+        double tx = 0, ty = 0;
+        for (int q = 0; q < 4; ++q) {
+            tx += op(sx + j * 4 + q, e) * l[q];
+            ty += op(sy + j * 4 + q, e) * l[q];
+        }
+        u -= tx;
+        v -= ty;
+    }
+    gu[n] = u / lumped1[n];
+    gv[n] = v / lumped1[n];
+}
+// Boundary extension of the CG1 gradient (CGDynamicsKernel.cpp:177-203) as an index map.
+__device__ __forceinline__ int sshSource(int nix, int niy, int nx, int ny)
+{
+    int sx = nix, sy = niy;
+    if (nix == 0)
+        sx = 1;
+    else if (nix == nx)
+        sx = nx - 1;
+    if (niy == 0)
+        sy = 1;
+    else if (niy == ny)
+        sy = ny - 1;
+    if ((nix == 0 || nix == nx) && (niy == 0 || niy == ny))
+        return sx + (nx + 1) * sy;
+    if (niy == 0 || niy == ny)
+        return nix + (nx + 1) * sy;
+    if (nix == 0 || nix == nx)
+        return sx + (nx + 1) * niy;
+    return nix + (nx + 1) * niy;
+}
+template <int CG> __global__ void k_ssh_prolong(int nx, int ny, const double* gu, const double* gv, double* gx, double* gy)
+{
+    const int row = CG * nx + 1, nn = row * (CG * ny + 1);
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    const int jx = n % row, jy = n / row;
+    auto src = [&](int a, int b) { return sshSource(a, b, nx, ny); };
+    if (CG == 1) {
+        const int s = src(jx, jy);
+        gx[n] = gu[s];
+        gy[n] = gv[s];
+        return;
+    }
+    const int ax = jx >> 1, ay = jy >> 1;
+    if (!(jx & 1) && !(jy & 1)) {
+        const int s = src(ax, ay);
+        gx[n] = gu[s];
+        gy[n] = gv[s];
+    } else if ((jx & 1) && !(jy & 1)) {
+        const int s0 = src(ax, ay), s1 = src(ax + 1, ay);
+        gx[n] = 0.5 * (gu[s0] + gu[s1]);
+        gy[n] = 0.5 * (gv[s0] + gv[s1]);
+    } else if (!(jx & 1) && (jy & 1)) {
+        const int s0 = src(ax, ay), s1 = src(ax, ay + 1);
+        gx[n] = 0.5 * (gu[s0] + gu[s1]);
+        gy[n] = 0.5 * (gv[s0] + gv[s1]);
+    } else {
+        const int s0 = src(ax, ay), s1 = src(ax + 1, ay), s2 = src(ax, ay + 1), s3 = src(ax + 1, ay + 1);
+        gx[n] = 0.25 * (gu[s0] + gu[s1] + gu[s2] + gu[s3]);
+        gy[n] = 0.25 * (gv[s0] + gv[s1] + gv[s2] + gv[s3]);
+    }
+}
+
+// ---------------------------------------------------------------- mEVP element pass
+struct ElemArgs {
+    int nx, ny, ne;
+    const uint8_t* ocean;
+    const double* u;
+    const double* v;
+    const double* h; // SoA DG
+    const double* A;
+    const double* eps; // SoA [3][S][ne] stored strain (land elements keep theirs)
+    double* epsOut;
+    double* sig; // SoA [3][S][ne]
+    const double* ops;
+    size_t stride;
+    double Pstar, DM2, invAlpha, relax;
+};
+
+// projectVelocityToStrain for one element (CGDynamicsKernel.cpp:300-337)
+template <int CG, bool UNI>
+__device__ __forceinline__ void strainLocal(const ElemArgs& a, int e, double* e11, double* e12, double* e22)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, N = L::N;
+    const Ops<UNI> op { a.ops, a.stride };
+    const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+    const int cgi = CG * row * iy + CG * ix;
+    double lu[N], lv[N];
+#pragma unroll
+    for (int r = 0; r <= CG; ++r)
+#pragma unroll
+        for (int c = 0; c <= CG; ++c) {
+            lu[c + (CG + 1) * r] = a.u[cgi + c + r * row];
+            lv[c + (CG + 1) * r] = a.v[cgi + c + r * row];
+        }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double x = 0, y = 0, p = 0, q = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const double gx = op(L::GX + s * N + j, e), gy = op(L::GY + s * N + j, e);
+            x += gx * lu[j];
+            y += gy * lv[j];
+            p += gx * lv[j];
+            q += gy * lu[j];
+        }
+        e11[s] = x;
+        e22[s] = y;
+        e12[s] = 0.5 * (p + q);
+    }
+}
+
+// stressUpdateHighOrder for one element (MEVPStressUpdateStep.hpp:30-118): sigma <-
+// (1 - 1/alpha) sigma + iMJwPSI (1/alpha) sigma_VP(strain, h, A) at the Gauss points.
+template <int CG, int DG, bool UNI>
+__device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const double* e11, const double* e12,
+    const double* e22, double* s11, double* s12, double* s22)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, NG = L::G, Q = L::Q;
+    const Ops<UNI> op { a.ops, a.stride };
+    const size_t ne = a.ne;
+    double hc[DG], ac[DG];
+#pragma unroll
+    for (int i = 0; i < DG; ++i) {
+        hc[i] = a.h[i * ne + e];
+        ac[i] = a.A[i * ne + e];
+    }
+    double acc1[S], acc2[S], acc3[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        acc1[s] = acc2[s] = acc3[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double hg = 0, ag = 0, g11 = 0, g12 = 0, g22 = 0;
+#pragma unroll
+        for (int i = 0; i < DG; ++i) {
+            hg += hc[i] * Psi<DG, NG>::v[i][q];
+            ag += ac[i] * Psi<DG, NG>::v[i][q];
+        }
+        hg = fmax(hg, 0.0);
+        ag = fmin(fmax(ag, 0.0), 1.0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            g11 += e11[s] * Psi<S, NG>::v[s][q];
+            g12 += e12[s] * Psi<S, NG>::v[s][q];
+            g22 += e22[s] * Psi<S, NG>::v[s][q];
+        }
+        const double delta = sqrt(a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12);
+        const double P = a.Pstar * hg * exp(-20.0 * (1.0 - ag));
+        const double c = (P * 0.125) / delta; // P/8/Delta
+        const double r11 = a.invAlpha * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
+        const double r12 = a.invAlpha * ((2.0 * c) * g12);
+        const double r22 = a.invAlpha * (c * (5.0 * g22 + 3.0 * g11) - 0.5 * P);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double b = op(L::B + s * Q + q, e);
+            acc1[s] += b * r11;
+            acc2[s] += b * r12;
+            acc3[s] += b * r22;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        s11[s] = s11[s] * a.relax + acc1[s];
+        s12[s] = s12[s] * a.relax + acc2[s];
+        s22[s] = s22[s] * a.relax + acc3[s];
+    }
+}
+
+template <int S> __device__ __forceinline__ void loadTensor(const double* f, size_t ne, int e, double* a, double* b, double* c)
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        a[s] = f[(0 * S + s) * ne + e];
+        b[s] = f[(1 * S + s) * ne + e];
+        c[s] = f[(2 * S + s) * ne + e];
+    }
+}
+template <int S> __device__ __forceinline__ void storeTensor(double* f, size_t ne, int e, const double* a, const double* b, const double* c)
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        f[(0 * S + s) * ne + e] = a[s];
+        f[(1 * S + s) * ne + e] = b[s];
+        f[(2 * S + s) * ne + e] = c[s];
+    }
+}
+
+// Fused strain + stress (one HBM pass per subcycle over the elements).
+template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_element(ElemArgs a)
+{
+    constexpr int S = cg2dgstress(CG);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne)
+        return;
+    double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
+    if (a.ocean[e])
+        strainLocal<CG, UNI>(a, e, e11, e12, e22);
+    else
+        loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
+    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
+    storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+}
+template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_strain(ElemArgs a)
+{
+    constexpr int S = cg2dgstress(CG);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne || !a.ocean[e])
+        return;
+    double e11[S], e12[S], e22[S];
+    strainLocal<CG, UNI>(a, e, e11, e12, e22);
+    storeTensor<S>(a.epsOut, a.ne, e, e11, e12, e22);
+}
+template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_stress(ElemArgs a)
+{
+    constexpr int S = cg2dgstress(CG);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne)
+        return;
+    double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
+    loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
+    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
+    storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+}
+
+// ---------------------------------------------------------------- mEVP node pass
+struct NodeArgs {
+    int nx, ny, ne, nn;
+    const uint8_t* ocean;
+    const uint8_t* dirNode;
+    const double* sig;
+    const double* ops;
+    size_t stride;
+    double* u;
+    double* v;
+    const double *u0, *v0, *uo, *vo, *ua, *va, *H, *A, *gx, *gy, *ML;
+    double* dSx;
+    double* dSy;
+    double rho_ice, F_atm, F_ocean, fc, gravity, beta, dt;
+};
+
+// updateMomentum for one node (VPCGDynamicsKernel.hpp:132-172), FP64.
+__device__ __forceinline__ void momentumNode(const NodeArgs& a, int n, double dSx, double dSy)
+{
+    const double uu = a.u[n], vv = a.v[n];
+    const double uOcnRel = a.uo[n] - uu, vOcnRel = vv - a.vo[n];
+    const double ua = a.ua[n], va = a.va[n];
+    const double absatm = sqrt(ua * ua + va * va);
+    const double absocn = sqrt(uOcnRel * uOcnRel + vOcnRel * vOcnRel);
+    const double H = a.H[n], A = a.A[n], ML = a.ML[n];
+    const double c1 = a.rho_ice * H / a.dt;
+    const double rhoH = a.rho_ice * H;
+    const double inv = 1.0 / (c1 * (1.0 + a.beta) + A * a.F_ocean * absocn);
+    a.u[n] = inv
+        * (c1 * (a.beta * uu + a.u0[n]) + A * (a.F_atm * absatm * ua + a.F_ocean * absocn * a.uo[n])
+            - rhoH * a.fc * uu - rhoH * a.gravity * a.gx[n] + dSx / ML);
+    a.v[n] = inv
+        * (c1 * (a.beta * vv + a.v0[n]) + A * (a.F_atm * absatm * va + a.F_ocean * absocn * a.vo[n])
+            + rhoH * a.fc * vv - rhoH * a.gravity * a.gy[n] + dSy / ML);
+}
+
+template <int CG> struct NodeTile {
+    static constexpr int TX = CG == 2 ? 32 : 16, TY = CG == 2 ? 8 : 16;
+    static constexpr int HX = TX + 1, HY = TY + 1, N = cgdofs(CG);
+};
+
+// Fused stressDivergence + updateMomentum + applyBoundaries (no periodic segments).
+template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
+{
+    using T = NodeTile<CG>;
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, N = T::N;
+    __shared__ double contrib[T::HX * T::HY][N][2];
+    const int tilesX = (a.nx + T::TX - 1) / T::TX;
+    const int ex0 = (blockIdx.x % tilesX) * T::TX, ey0 = (blockIdx.x / tilesX) * T::TY;
+    const Ops<UNI> op { a.ops, a.stride };
+    const size_t ne = a.ne;
+
+    // Phase 1: weak-divergence contributions of the tile and its left/bottom halo.
+    for (int le = threadIdx.x; le < T::HX * T::HY; le += kBlock) {
+        const int ex = ex0 - 1 + le % T::HX, ey = ey0 - 1 + le / T::HX;
+        double tx[N], ty[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            tx[j] = ty[j] = 0.0;
+        if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
+            const int e = ex + a.nx * ey;
+            if (a.ocean[e]) {
+                double s11[S], s12[S], s22[S];
+                loadTensor<S>(a.sig, ne, e, s11, s12, s22);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    double p1 = 0, p2 = 0, p3 = 0, p4 = 0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const double d1 = op(L::D1 + j * S + s, e), d2 = op(L::D2 + j * S + s, e);
+                        p1 += d1 * s11[s];
+                        p2 += d2 * s12[s];
+                        p3 += d1 * s12[s];
+                        p4 += d2 * s22[s];
+                    }
+                    tx[j] = p1 + p2;
+                    ty[j] = p3 + p4;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            contrib[le][j][0] = tx[j];
+            contrib[le][j][1] = ty[j];
+        }
+    }
+    __syncthreads();
+
+    // Phase 2: nodes owned by the tile (the last tile row/column also owns the domain edge).
+    const int row = CG * a.nx + 1;
+    const int nx0 = CG * ex0, ny0 = CG * ey0;
+    const int nx1 = (ex0 + T::TX >= a.nx) ? CG * a.nx + 1 : CG * (ex0 + T::TX);
+    const int ny1 = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 : CG * (ey0 + T::TY);
+    const int wN = nx1 - nx0, cnt = wN * (ny1 - ny0);
+    for (int ln = threadIdx.x; ln < cnt; ln += kBlock) {
+        const int nix = nx0 + ln % wN, niy = ny0 + ln / wN;
+        const int n = nix + row * niy;
+        if (a.dirNode[n]) {
+            a.u[n] = 0.0;
+            a.v[n] = 0.0;
+            continue;
+        }
+        const Adj adj = adjacent<CG>(nix, niy, a.nx, a.ny, true);
+        double dx = 0.0, dy = 0.0;
+        for (int k = 0; k < adj.n; ++k) {
+            const int e = adj.e[k];
+            if (!a.ocean[e])
+                continue;
+            const int le = (e % a.nx - ex0 + 1) + T::HX * (e / a.nx - ey0 + 1);
+            dx -= contrib[le][adj.loc[k]][0];
+            dy -= contrib[le][adj.loc[k]][1];
+        }
+        momentumNode(a, n, dx, dy);
+    }
+}
+
+// Split passes (periodic meshes and per-pass testing).
+template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_divergence(NodeArgs a)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nn)
+        return;
+    if (a.dirNode[n]) {
+        a.dSx[n] = 0.0;
+        a.dSy[n] = 0.0;
+        return;
+    }
+    const Ops<UNI> op { a.ops, a.stride };
+    const int row = CG * a.nx + 1;
+    const Adj adj = adjacent<CG>(n % row, n / row, a.nx, a.ny, true);
+    double dx = 0.0, dy = 0.0;
+    for (int k = 0; k < adj.n; ++k) {
+        const int e = adj.e[k], j = adj.loc[k];
+        if (!a.ocean[e])
+            continue;
+        double s11[S], s12[S], s22[S];
+        loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+        double p1 = 0, p2 = 0, p3 = 0, p4 = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double d1 = op(L::D1 + j * S + s, e), d2 = op(L::D2 + j * S + s, e);
+            p1 += d1 * s11[s];
+            p2 += d2 * s12[s];
+            p3 += d1 * s12[s];
+            p4 += d2 * s22[s];
+        }
+        dx -= p1 + p2;
+        dy -= p3 + p4;
+    }
+    a.dSx[n] = dx;
+    a.dSy[n] = dy;
+}
+__global__ void k_momentum(NodeArgs a)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nn)
+        return;
+    momentumNode(a, n, a.dSx[n], a.dSy[n]);
+}
+__global__ void k_boundaries(NodeArgs a)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nn || !a.dirNode[n])
+        return;
+    a.u[n] = 0.0;
+    a.v[n] = 0.0;
+}
+
+// ---------------------------------------------------------------- outputs and inputs
+// getIceOceanStress (VPCGDynamicsKernel.hpp:96-120)
+__global__ void k_ice_ocean_stress(int nn, double F_ocean, const double* u, const double* v, const double* uo,
+    const double* vo, double* tx, double* ty)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    const double ur = u[n] - uo[n], vr = v[n] - vo[n];
+    const double a = sqrt(ur * ur + vr * vr);
+    tx[n] = F_ocean * a * ur;
+    ty[n] = F_ocean * a * vr;
+}
+// Benchmark cyclone (BenchmarkAtmosphere.cpp:29-76) at the cell lower-left corners.
+__global__ void k_benchmark_wind(int nx, int ny, double dx, double dy, double x0, double y0, double vmax, double* uw,
+    double* vw)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nx * ny)
+        return;
+    const double A = 1e-5, k = 1e-5;
+    const double alpha = 72. / 180. * M_PI;
+    const double ca = cos(alpha), sa = sin(alpha);
+    const double xp = (e % nx) * dx - x0, yp = (e / nx) * dy - y0;
+    const double scale = A * exp(-k * hypot(xp, yp));
+    uw[e] = -scale * vmax * (ca * xp + sa * yp);
+    vw[e] = -scale * vmax * (-sa * xp + ca * yp);
+}
+__global__ void k_zero_higher(double* f, int ne, int nc)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<size_t>(ne) * (nc - 1))
+        return;
+    f[ne + t] = 0.0;
+}
+
+// Compensated (two-sum) block reduction of the element integrals of h and A
+// (Tools::MeanValue, Tools.hpp:24-45: 3x3 Gauss, J w weights, every element).
+__device__ __forceinline__ void twoSum(double a, double b, double& s, double& err)
+{
+    s = a + b;
+    const double bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+}
+template <int DG>
+__global__ __launch_bounds__(kBlock) void k_mass_partial(MeshView m, int ne, const double* h, const double* A, double* part)
+{
+    __shared__ double sh[4][kBlock];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double vh = 0, va = 0;
+    if (e < ne) {
+        double c[4][2], J[9];
+        elementCoords(m, e, c);
+        jacobian<3>(c, J);
+        for (int q = 0; q < 9; ++q) {
+            double ph = 0, pa = 0;
+            for (int i = 0; i < DG; ++i) {
+                ph += h[static_cast<size_t>(i) * ne + e] * Psi<DG, 3>::v[i][q];
+                pa += A[static_cast<size_t>(i) * ne + e] * Psi<DG, 3>::v[i][q];
+            }
+            vh += ph * (J[q] * W<3>::v[0][q]);
+            va += pa * (J[q] * W<3>::v[0][q]);
+        }
+    }
+    sh[0][threadIdx.x] = vh;
+    sh[1][threadIdx.x] = 0.0;
+    sh[2][threadIdx.x] = va;
+    sh[3][threadIdx.x] = 0.0;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int f = 0; f < 2; ++f) {
+                double s, err;
+                twoSum(sh[2 * f][threadIdx.x], sh[2 * f][threadIdx.x + w], s, err);
+                sh[2 * f][threadIdx.x] = s;
+                sh[2 * f + 1][threadIdx.x] += sh[2 * f + 1][threadIdx.x + w] + err;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int f = 0; f < 4; ++f)
+            part[4 * blockIdx.x + f] = sh[f][0];
+}
+__global__ void k_mass_final(const double* part, int nb, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    for (int f = 0; f < 2; ++f) {
+        double s = 0, c = 0;
+        for (int b = 0; b < nb; ++b) {
+            double t, err;
+            twoSum(s, part[4 * b + 2 * f], t, err);
+            s = t;
+            c += err + part[4 * b + 2 * f + 1];
+        }
+        out[f] = s + c;
+    }
+}
+
+// ---------------------------------------------------------------- host object
+template <int CG, int DG> struct DynImpl final : DynBase {
+    using L = MomLayout<CG>;
+    static constexpr int S = L::S, N = L::N;
+    nsdg_params p;
+    hipStream_t s = nullptr;
+    Profiler prof;
+    DeviceMesh mesh;
+    std::unique_ptr<TransportBase> tr;
+    CG2DGOps cg2dg;
+    bool ready = false;
+    int precision = NSDG_PREC_FP64;
+    size_t ne = 0, nn = 0, nn1 = 0;
+    double deltaT = 0;
+    DBuf<double> mops, lumped, lumped1, h, A, ssh, eps, sig;
+    DBuf<double> u, v, u0, v0, uo, vo, ua, va, cgH, cgA, gX, gY, dSx, dSy;
+    DBuf<double> tmpDG, tmpDG2, tmpCG, tmpCG2, cg1a, cg1b, cg1c, scratch, dg0a, dg0b, part, res;
+    DBuf<uint8_t> dirNode;
+    std::map<std::string, std::pair<int, std::vector<double>>> passive;
+
+    explicit DynImpl(const nsdg_params& prm)
+        : p(prm)
+    {
+        NSDG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    }
+    ~DynImpl() override
+    {
+        tr.reset();
+        if (s)
+            (void)hipStreamDestroy(s);
+    }
+    Profiler& profiler() override { return prof; }
+    bool initialised() const override { return ready; }
+    size_t cgNodes() const override { return nn; }
+    void need() const { require(ready, "kernel not initialised", NSDG_ERR_STATE); }
+    size_t opStride() const { return mesh.uniform ? 1 : ne; }
+
+    void initialise(const HostMesh& m) override
+    {
+        ready = false;
+        mesh.build(m, s);
+        ne = mesh.ne;
+        nn = static_cast<size_t>(CG * mesh.nx + 1) * (CG * mesh.ny + 1);
+        nn1 = static_cast<size_t>(mesh.nx + 1) * (mesh.ny + 1);
+        tr = makeTransport(DG, mesh, s, prof);
+        tr->setScheme(2); // DynamicsKernel.hpp:61
+        cg2dg.build(CG, DG, mesh, s);
+        const int nOps = mesh.uniform ? 1 : static_cast<int>(ne);
+        mops.alloc(static_cast<size_t>(L::K) * nOps);
+        k_momentum_ops<CG><<<blocksFor(nOps), kBlock, 0, s>>>(view(mesh), nOps, mops.p);
+        checkLaunch("momentum_ops");
+        lumped.alloc(nn);
+        lumped1.alloc(nn1);
+        k_lumped_mass<CG, CG == 1 ? 1 : 4><<<blocksFor(nn), kBlock, 0, s>>>(view(mesh), lumped.p);
+        checkLaunch("lumped_mass");
+        k_lumped_mass<1, 2><<<blocksFor(nn1), kBlock, 0, s>>>(view(mesh), lumped1.p);
+        checkLaunch("lumped_mass1");
+        for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
+            b->alloc(ne * DG);
+        ssh.alloc(ne);
+        dg0a.alloc(ne);
+        dg0b.alloc(ne);
+        eps.alloc(3 * S * ne);
+        sig.alloc(3 * S * ne);
+        for (auto* b : { &u, &v, &u0, &v0, &uo, &vo, &ua, &va, &cgH, &cgA, &gX, &gY, &dSx, &dSy, &tmpCG, &tmpCG2 })
+            b->alloc(nn);
+        for (auto* b : { &cg1a, &cg1b, &cg1c })
+            b->alloc(nn1);
+        part.alloc(4 * blocksFor(ne));
+        res.alloc(2);
+        for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
+            deviceFill(b->p, b->n, 0.0, s);
+        for (auto* b : { &ssh, &eps, &sig, &u, &v, &u0, &v0, &uo, &vo, &ua, &va, &cgH, &cgA, &gX, &gY, &dSx, &dSy })
+            deviceFill(b->p, b->n, 0.0, s);
+        // Dirichlet CG nodes (dirichletZero, CGDynamicsKernel.cpp:400-437)
+        std::vector<uint8_t> dn(nn, 0);
+        const size_t nx = mesh.host.nx, row = CG * nx + 1;
+        for (int seg = 0; seg < 4; ++seg)
+            for (auto eid : mesh.host.dirichlet[seg]) {
+                const size_t ix = eid % nx, iy = eid / nx;
+                for (size_t j = 0; j <= CG; ++j) {
+                    size_t id;
+                    if (seg == 0)
+                        id = iy * CG * row + CG * ix + j;
+                    else if (seg == 1)
+                        id = iy * CG * row + CG * ix + CG + row * j;
+                    else if (seg == 2)
+                        id = (iy + 1) * CG * row + CG * ix + j;
+                    else
+                        id = iy * CG * row + CG * ix + row * j;
+                    dn[id] = 1;
+                }
+            }
+        dirNode.alloc(nn);
+        dirNode.upload(dn.data(), nn, s);
+        passive.clear();
+        NSDG_HIP(hipStreamSynchronize(s));
+        ready = true;
+    }
+
+    // ---- field access
+    double* cgRef(const std::string& n)
+    {
+        if (n == "u") return u.p;
+        if (n == "v") return v.p;
+        if (n == "u0") return u0.p;
+        if (n == "v0") return v0.p;
+        if (n == "cgH") return cgH.p;
+        if (n == "cgA") return cgA.p;
+        if (n == "gradX") return gX.p;
+        if (n == "gradY") return gY.p;
+        if (n == "dStressX") return dSx.p;
+        if (n == "dStressY") return dSy.p;
+        if (n == "uocean") return uo.p;
+        if (n == "vocean") return vo.p;
+        if (n == "uwind") return ua.p;
+        if (n == "vwind") return va.p;
+        if (n == "lumpedcgmass") return lumped.p;
+        return nullptr;
+    }
+    double* tensorRef(const std::string& n)
+    {
+        if (n == "s11") return sig.p;
+        if (n == "s12") return sig.p + S * ne;
+        if (n == "s22") return sig.p + 2 * S * ne;
+        if (n == "e11") return eps.p;
+        if (n == "e12") return eps.p + S * ne;
+        if (n == "e22") return eps.p + 2 * S * ne;
+        return nullptr;
+    }
+    // DG2CG of a DG0 host field into a CG node vector (setData of u, v, forcing).
+    void dg0ToCG(const double* data, int ncomp, double* dst)
+    {
+        if (ncomp == DG) {
+            uploadSoA(scratch, data, ne, DG, tmpDG.p, s);
+            launchDG2CG<DG>(tmpDG.p, dst, 0, nullptr, nullptr, 0);
+        } else {
+            require(ncomp == 1, "field must have 1 or DG components");
+            dg0a.upload(data, ne, s);
+            launchDG2CG<1>(dg0a.p, dst, 0, nullptr, nullptr, 0);
+        }
+    }
+    template <int D> void launchDG2CG(const double* s1, double* d1, int c1, const double* s2, double* d2, int c2)
+    {
+        Timed t(prof, s, "dg2cg");
+        k_dg2cg<CG, D><<<blocksFor(nn), kBlock, 0, s>>>(mesh.nx, mesh.ny, static_cast<int>(ne), s1, d1, c1, s2, d2, c2);
+        checkLaunch("dg2cg");
+    }
+    void setField(const std::string& n, const double* data, int ncomp) override
+    {
+        need();
+        require(data != nullptr, "data is NULL");
+        if (double* c = (n == "u" || n == "v" || n == "uwind" || n == "vwind" || n == "uocean" || n == "vocean")
+                ? cgRef(n)
+                : nullptr) {
+            dg0ToCG(data, ncomp, c);
+        } else if (n == "hice" || n == "cice") {
+            double* f = n == "hice" ? h.p : A.p;
+            require(ncomp == 1 || ncomp == DG, "hice/cice need 1 or DG components");
+            if (ncomp == DG)
+                uploadSoA(scratch, data, ne, DG, f, s);
+            else {
+                deviceFill(f, ne * DG, 0.0, s);
+                NSDG_HIP(hipMemcpyAsync(f, data, ne * sizeof(double), hipMemcpyHostToDevice, s));
+            }
+        } else if (n == "ssh") {
+            require(ncomp == 1, "ssh is a DG0 field");
+            ssh.upload(data, ne, s);
+        } else if (double* t = tensorRef(n)) {
+            require(ncomp == 1 || ncomp == S, "stress/strain need 1 or S components");
+            if (ncomp == S)
+                uploadSoA(scratch, data, ne, S, t, s);
+            else {
+                deviceFill(t, ne * S, 0.0, s);
+                NSDG_HIP(hipMemcpyAsync(t, data, ne * sizeof(double), hipMemcpyHostToDevice, s));
+            }
+        } else {
+            require(ncomp >= 1, "ncomp must be positive");
+            passive[n] = { ncomp, std::vector<double>(data, data + ne * ncomp) };
+        }
+        NSDG_HIP(hipStreamSynchronize(s));
+    }
+    void getField(const std::string& n, double* out, int ncomp) override
+    {
+        need();
+        require(out != nullptr, "out is NULL");
+        if (n == "u" || n == "v" || n == "uiostress" || n == "viostress") {
+            require(ncomp == 1 || ncomp == DG, "ncomp must be 1 or DG");
+            const double* src = n == "u" ? u.p : v.p;
+            if (n == "uiostress" || n == "viostress") {
+                k_ice_ocean_stress<<<blocksFor(nn), kBlock, 0, s>>>(
+                    static_cast<int>(nn), p.F_ocean, u.p, v.p, uo.p, vo.p, tmpCG.p, tmpCG2.p);
+                checkLaunch("ice_ocean_stress");
+                src = n == "uiostress" ? tmpCG.p : tmpCG2.p;
+            }
+            cg2dg.apply(mesh, src, tmpDG.p, nullptr, nullptr, s, prof);
+            downloadSoA(scratch, tmpDG.p, ne, ncomp == 1 ? 1 : DG, out, s);
+        } else if (n == "hice" || n == "cice") {
+            require(ncomp == 1 || ncomp == DG, "ncomp must be 1 or DG");
+            downloadSoA(scratch, n == "hice" ? h.p : A.p, ne, ncomp, out, s);
+        } else if (n == "ssh") {
+            require(ncomp == 1, "ssh is a DG0 field");
+            downloadSoA(scratch, ssh.p, ne, 1, out, s);
+        } else if (const double* t = tensorRef(n)) {
+            require(ncomp == 1 || ncomp == S, "ncomp must be 1 or S");
+            downloadSoA(scratch, t, ne, ncomp, out, s);
+        } else {
+            auto it = passive.find(n);
+            require(it != passive.end(), "unknown field " + n);
+            const int nc = it->second.first;
+            const auto& d = it->second.second;
+            if (ncomp == nc)
+                std::memcpy(out, d.data(), d.size() * sizeof(double));
+            else {
+                require(ncomp == 1, "ncomp must be 1 or the stored component count");
+                for (size_t e = 0; e < ne; ++e)
+                    out[e] = d[e * nc];
+            }
+        }
+    }
+    void getCG(const std::string& n, double* out) override
+    {
+        need();
+        double* c = cgRef(n);
+        require(c != nullptr, "unknown CG vector " + n);
+        NSDG_HIP(hipMemcpyAsync(out, c, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+        NSDG_HIP(hipStreamSynchronize(s));
+    }
+    void setCG(const std::string& n, const double* in) override
+    {
+        need();
+        double* c = cgRef(n);
+        require(c != nullptr && n != "lumpedcgmass", "unknown or read-only CG vector " + n);
+        NSDG_HIP(hipMemcpyAsync(c, in, nn * sizeof(double), hipMemcpyHostToDevice, s));
+        NSDG_HIP(hipStreamSynchronize(s));
+    }
+
+    // ---- the time step
+    void advectAsync(double dt)
+    {
+        cg2dg.apply(mesh, u.p, tr->velx(), v.p, tr->vely(), s, prof);
+        tr->reinitNormalVelocity();
+        tr->step(dt, A.p);
+        tr->step(dt, h.p);
+        tr->limit(A.p, true, 1.0, true, 0.0);
+        tr->limit(h.p, false, 0.0, true, 0.0);
+    }
+    void prepareIterationAsync()
+    {
+        const int clampH = mesh.hasPeriodic ? 0 : 1, clampA = mesh.hasPeriodic ? 0 : 2;
+        launchDG2CG<DG>(h.p, cgH.p, clampH, A.p, cgA.p, clampA);
+        if (mesh.hasPeriodic) {
+            for (const auto& seg : mesh.segments) {
+                k_periodic_average<CG><<<1, 1, 0, s>>>(
+                    mesh.nx, mesh.periodic.p + 4 * seg.first, static_cast<int>(seg.second), cgH.p, cgA.p);
+                checkLaunch("periodic_average");
+            }
+        }
+        // sea-surface height gradient
+        {
+            Timed t(prof, s, "ssh_gradient");
+            k_dg2cg<1, 1><<<blocksFor(nn1), kBlock, 0, s>>>(
+                mesh.nx, mesh.ny, static_cast<int>(ne), ssh.p, cg1a.p, 0, nullptr, nullptr, 0);
+            checkLaunch("ssh_dg2cg");
+            auto go = [&](auto kern) {
+                kern<<<blocksFor(nn1), kBlock, 0, s>>>(
+                    mesh.nx, mesh.ny, cg1a.p, mops.p, opStride(), L::SX, L::SY, lumped1.p, cg1b.p, cg1c.p);
+                checkLaunch("ssh_gradient");
+            };
+            if (mesh.uniform)
+                go(k_ssh_gradient<true>);
+            else
+                go(k_ssh_gradient<false>);
+            k_ssh_prolong<CG><<<blocksFor(nn), kBlock, 0, s>>>(mesh.nx, mesh.ny, cg1b.p, cg1c.p, gX.p, gY.p);
+            checkLaunch("ssh_prolong");
+        }
+        if (mesh.hasPeriodic) {
+            k_clamp_HA<CG><<<blocksFor(nn), kBlock, 0, s>>>(cgH.p, cgA.p, static_cast<int>(nn));
+            checkLaunch("clamp");
+        }
+    }
+    ElemArgs elemArgs()
+    {
+        ElemArgs a {};
+        a.nx = mesh.nx;
+        a.ny = mesh.ny;
+        a.ne = static_cast<int>(ne);
+        a.ocean = mesh.ocean.p;
+        a.u = u.p;
+        a.v = v.p;
+        a.h = h.p;
+        a.A = A.p;
+        a.eps = eps.p;
+        a.epsOut = eps.p;
+        a.sig = sig.p;
+        a.ops = mops.p;
+        a.stride = opStride();
+        a.Pstar = p.Pstar;
+        a.DM2 = p.DeltaMin * p.DeltaMin;
+        a.invAlpha = 1.0 / p.alpha;
+        a.relax = 1.0 - 1.0 / p.alpha;
+        return a;
+    }
+    NodeArgs nodeArgs()
+    {
+        NodeArgs a {};
+        a.nx = mesh.nx;
+        a.ny = mesh.ny;
+        a.ne = static_cast<int>(ne);
+        a.nn = static_cast<int>(nn);
+        a.ocean = mesh.ocean.p;
+        a.dirNode = dirNode.p;
+        a.sig = sig.p;
+        a.ops = mops.p;
+        a.stride = opStride();
+        a.u = u.p;
+        a.v = v.p;
+        a.u0 = u0.p;
+        a.v0 = v0.p;
+        a.uo = uo.p;
+        a.vo = vo.p;
+        a.ua = ua.p;
+        a.va = va.p;
+        a.H = cgH.p;
+        a.A = cgA.p;
+        a.gx = gX.p;
+        a.gy = gY.p;
+        a.ML = lumped.p;
+        a.dSx = dSx.p;
+        a.dSy = dSy.p;
+        a.rho_ice = p.rho_ice;
+        a.F_atm = p.F_atm;
+        a.F_ocean = p.F_ocean;
+        a.fc = p.fc;
+        a.gravity = p.gravity;
+        a.beta = p.beta;
+        a.dt = deltaT;
+        return a;
+    }
+    void elementPass()
+    {
+        Timed t(prof, s, "mevp_element");
+        const ElemArgs a = elemArgs();
+        if (mesh.uniform)
+            k_mevp_element<CG, DG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        else
+            k_mevp_element<CG, DG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        checkLaunch("mevp_element");
+    }
+    void nodePass()
+    {
+        const NodeArgs a = nodeArgs();
+        if (!mesh.hasPeriodic) {
+            Timed t(prof, s, "mevp_node");
+            using T = NodeTile<CG>;
+            const unsigned tiles = static_cast<unsigned>(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+            if (mesh.uniform)
+                k_mevp_node<CG, true><<<tiles, kBlock, 0, s>>>(a);
+            else
+                k_mevp_node<CG, false><<<tiles, kBlock, 0, s>>>(a);
+            checkLaunch("mevp_node");
+            return;
+        }
+        divergenceAsync();
+        momentumAsync();
+        boundariesAsync();
+    }
+    void divergenceAsync()
+    {
+        Timed t(prof, s, "divergence");
+        const NodeArgs a = nodeArgs();
+        if (mesh.uniform)
+            k_divergence<CG, true><<<blocksFor(nn), kBlock, 0, s>>>(a);
+        else
+            k_divergence<CG, false><<<blocksFor(nn), kBlock, 0, s>>>(a);
+        checkLaunch("divergence");
+        for (const auto& seg : mesh.segments) {
+            k_periodic_average<CG><<<1, 1, 0, s>>>(
+                mesh.nx, mesh.periodic.p + 4 * seg.first, static_cast<int>(seg.second), dSx.p, dSy.p);
+            checkLaunch("periodic_average");
+        }
+    }
+    void momentumAsync()
+    {
+        Timed t(prof, s, "momentum");
+        k_momentum<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs());
+        checkLaunch("momentum");
+    }
+    void boundariesAsync()
+    {
+        k_boundaries<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs());
+        checkLaunch("boundaries");
+    }
+    void beginIterationAsync(double dt)
+    {
+        deviceCopy(u0.p, u.p, nn, s);
+        deviceCopy(v0.p, v.p, nn, s);
+        deltaT = dt;
+    }
+    void subcyclesAsync(int n)
+    {
+        for (int k = 0; k < n; ++k) {
+            elementPass();
+            nodePass();
+        }
+    }
+
+    void update(double dt) override
+    {
+        need();
+        require(dt > 0, "dt must be positive");
+        advectAsync(dt);
+        prepareIterationAsync();
+        beginIterationAsync(dt);
+        subcyclesAsync(p.nsteps);
+    }
+    void advect(double dt) override
+    {
+        need();
+        advectAsync(dt);
+        synchronize();
+    }
+    void prepareIteration() override
+    {
+        need();
+        prepareIterationAsync();
+        synchronize();
+    }
+    void beginIteration(double dt) override
+    {
+        need();
+        require(dt > 0, "dt must be positive");
+        beginIterationAsync(dt);
+        synchronize();
+    }
+    void strain() override
+    {
+        need();
+        const ElemArgs a = elemArgs();
+        if (mesh.uniform)
+            k_strain<CG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        else
+            k_strain<CG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        checkLaunch("strain");
+        synchronize();
+    }
+    void stress(int repeat) override
+    {
+        need();
+        const ElemArgs a = elemArgs();
+        for (int r = 0; r < repeat; ++r) {
+            if (mesh.uniform)
+                k_stress<CG, DG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
+            else
+                k_stress<CG, DG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
+            checkLaunch("stress");
+        }
+        synchronize();
+    }
+    void divergence() override
+    {
+        need();
+        divergenceAsync();
+        synchronize();
+    }
+    void momentum() override
+    {
+        need();
+        require(deltaT > 0, "begin_iteration must set dt first", NSDG_ERR_STATE);
+        momentumAsync();
+        synchronize();
+    }
+    void boundaries() override
+    {
+        need();
+        boundariesAsync();
+        synchronize();
+    }
+    void subcycles(int n) override
+    {
+        need();
+        require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
+        subcyclesAsync(n);
+    }
+    void setPrecision(int pr) override
+    {
+        require(pr == NSDG_PREC_FP64 || pr == NSDG_PREC_FP32, "precision must be NSDG_PREC_FP64 or NSDG_PREC_FP32");
+        require(pr == NSDG_PREC_FP64, "FP32 element pass not built", NSDG_ERR_UNSUPPORTED);
+        precision = pr;
+    }
+    void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) override
+    {
+        need();
+        const double dx = L / mesh.nx, dy = L / mesh.ny, tf = t / 86400.0;
+        const double x0 = mesh.nx * dx * 0.5 * (1 + tf / 5.0) + xoff;
+        const double y0 = mesh.ny * dy * 0.5 * (1 + tf / 5.0) + yoff;
+        {
+            Timed tm(prof, s, "benchmark_wind");
+            k_benchmark_wind<<<blocksFor(ne), kBlock, 0, s>>>(mesh.nx, mesh.ny, dx, dy, x0, y0, vmax, dg0a.p, dg0b.p);
+            checkLaunch("benchmark_wind");
+        }
+        launchDG2CG<1>(dg0a.p, ua.p, 0, dg0b.p, va.p, 0);
+    }
+    void resetDG0(const std::string& n) override
+    {
+        need();
+        double* f = nullptr;
+        int nc = 0;
+        if (n == "hice" || n == "cice") {
+            f = n == "hice" ? h.p : A.p;
+            nc = DG;
+        } else if (double* t = tensorRef(n)) {
+            f = t;
+            nc = S;
+        }
+        require(f != nullptr, "reset_dg0 applies to hice, cice and the tensors");
+        if (nc > 1) {
+            k_zero_higher<<<blocksFor(ne * (nc - 1)), kBlock, 0, s>>>(f, static_cast<int>(ne), nc);
+            checkLaunch("zero_higher");
+        }
+    }
+    void mass(double* vol, double* area) override
+    {
+        need();
+        const unsigned nb = blocksFor(ne);
+        k_mass_partial<DG><<<nb, kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), h.p, A.p, part.p);
+        checkLaunch("mass_partial");
+        k_mass_final<<<1, 1, 0, s>>>(part.p, static_cast<int>(nb), res.p);
+        checkLaunch("mass_final");
+        double r[2];
+        res.download(r, 2, s);
+        NSDG_HIP(hipStreamSynchronize(s));
+        if (vol)
+            *vol = r[0];
+        if (area)
+            *area = r[1];
+    }
+    void synchronize() override
+    {
+        NSDG_HIP(hipStreamSynchronize(s));
+        if (prof.on)
+            prof.collect();
+    }
+
+    // ---- state blob: header, then hice, cice (DG), sigma (3S), strain (3S), u, v (CG)
+    struct Header {
+        char magic[8];
+        int32_t version, cg, dg, s;
+        uint64_t nx, ny;
+        double deltaT;
+    };
+    std::vector<std::pair<double*, size_t>> stateParts()
+    {
+        return { { h.p, ne * DG }, { A.p, ne * DG }, { sig.p, 3 * S * ne }, { eps.p, 3 * S * ne }, { u.p, nn },
+            { v.p, nn }, { u0.p, nn }, { v0.p, nn } };
+    }
+    std::vector<char> saveState() override
+    {
+        need();
+        size_t total = sizeof(Header);
+        for (auto& pr : stateParts())
+            total += pr.second * sizeof(double);
+        std::vector<char> buf(total);
+        Header hd {};
+        std::memcpy(hd.magic, "NSDGST01", 8);
+        hd.version = 1;
+        hd.cg = CG;
+        hd.dg = DG;
+        hd.s = S;
+        hd.nx = mesh.nx;
+        hd.ny = mesh.ny;
+        hd.deltaT = deltaT;
+        std::memcpy(buf.data(), &hd, sizeof hd);
+        size_t off = sizeof(Header);
+        for (auto& pr : stateParts()) {
+            NSDG_HIP(hipMemcpyAsync(buf.data() + off, pr.first, pr.second * sizeof(double), hipMemcpyDeviceToHost, s));
+            off += pr.second * sizeof(double);
+        }
+        NSDG_HIP(hipStreamSynchronize(s));
+        return buf;
+    }
+    void loadState(const char* buf, size_t size) override
+    {
+        need();
+        require(size >= sizeof(Header), "state blob too small");
+        Header hd;
+        std::memcpy(&hd, buf, sizeof hd);
+        require(std::memcmp(hd.magic, "NSDGST01", 8) == 0 && hd.version == 1, "not a state blob of this library");
+        require(hd.cg == CG && hd.dg == DG && hd.nx == static_cast<uint64_t>(mesh.nx)
+                && hd.ny == static_cast<uint64_t>(mesh.ny),
+            "state blob does not match this kernel's degrees or mesh");
+        size_t off = sizeof(Header);
+        for (auto& pr : stateParts()) {
+            require(off + pr.second * sizeof(double) <= size, "state blob truncated");
+            NSDG_HIP(hipMemcpyAsync(pr.first, buf + off, pr.second * sizeof(double), hipMemcpyHostToDevice, s));
+            off += pr.second * sizeof(double);
+        }
+        deltaT = hd.deltaT;
+        NSDG_HIP(hipStreamSynchronize(s));
+    }
+};
+
+std::unique_ptr<DynBase> makeDyn(int cg, int dg, const nsdg_params& p)
+{
+    if (cg == 1 && dg == 3)
+        return std::make_unique<DynImpl<1, 3>>(p);
+    if (cg == 2 && dg == 6)
+        return std::make_unique<DynImpl<2, 6>>(p);
+    throw Error(NSDG_ERR_UNSUPPORTED, "supported (cg_degree, dg_adv) pairs: (1, 3) and (2, 6)");
+}
+
+} // namespace nsdg
