@@ -153,66 +153,59 @@ template <int DG, int SIDE> __device__ __forceinline__ void trace(const double* 
 }
 
 // ---------------------------------------------------------------- node adjacency
-// Elements sharing CG node (nix, niy) of a degree-CG mesh, in the order a two-colour
-// row-stripe scatter visits them: rows of one parity first, then the other, columns
-// ascending. loc is the element-local CG dof jx + (CG+1) jy of the node.
-struct Adj {
-    int n;
-    int e[4];
-    int loc[4];
-};
-template <int CG> __device__ __forceinline__ Adj adjacent(int nix, int niy, int nx, int ny, bool evenRowsFirst)
+// Elements sharing CG node (nix, niy) of a degree-CG mesh, visited in the order a
+// two-colour row-stripe scatter adds them: element rows of one parity first, then the
+// other, columns ascending. f(e, loc) receives the element and the node's local CG dof
+// jx + (CG+1) jy. Fully unrolled: every candidate lives in registers.
+__device__ __forceinline__ void axisCandidates(int cg, int ni, int n, int& c0, int& l0, bool& h0, int& c1, int& l1,
+    bool& h1)
 {
-    int ex[2], lx[2], cx = 0, ey[2], ly[2], cy = 0;
-    if (CG == 2 && (nix & 1)) {
-        ex[0] = nix >> 1;
-        lx[0] = 1;
-        cx = 1;
-    } else {
-        const int c = nix / CG;
-        if (c > 0) {
-            ex[cx] = c - 1;
-            lx[cx++] = CG;
-        }
-        if (c < nx) {
-            ex[cx] = c;
-            lx[cx++] = 0;
+    if (cg == 2 && (ni & 1)) {
+        c0 = ni >> 1;
+        l0 = 1;
+        h0 = true;
+        c1 = 0;
+        l1 = 0;
+        h1 = false;
+        return;
+    }
+    const int c = ni / cg;
+    c0 = c - 1;
+    l0 = cg;
+    h0 = c > 0;
+    c1 = c;
+    l1 = 0;
+    h1 = c < n;
+}
+template <int CG, bool EVEN_ROWS_FIRST, class F>
+__device__ __forceinline__ void forEachAdjacent(int nix, int niy, int nx, int ny, F&& f)
+{
+    int xa, la, xb, lb, ya, ma, yb, mb;
+    bool ha, hb, ga, gb;
+    axisCandidates(CG, nix, nx, xa, la, ha, xb, lb, hb);
+    axisCandidates(CG, niy, ny, ya, ma, ga, yb, mb, gb);
+    if (ga && gb && (((ya & 1) == 0) != EVEN_ROWS_FIRST)) {
+        const int ty = ya, tm = ma;
+        ya = yb;
+        ma = mb;
+        yb = ty;
+        mb = tm;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const bool hr = r == 0 ? ga : gb;
+        const int ey = r == 0 ? ya : yb, my = r == 0 ? ma : mb;
+        if (!hr)
+            continue;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bool hc = c == 0 ? ha : hb;
+            const int ex = c == 0 ? xa : xb, mx = c == 0 ? la : lb;
+            if (!hc)
+                continue;
+            f(ex + nx * ey, mx + (CG + 1) * my);
         }
     }
-    if (CG == 2 && (niy & 1)) {
-        ey[0] = niy >> 1;
-        ly[0] = 1;
-        cy = 1;
-    } else {
-        const int c = niy / CG;
-        if (c > 0) {
-            ey[cy] = c - 1;
-            ly[cy++] = CG;
-        }
-        if (c < ny) {
-            ey[cy] = c;
-            ly[cy++] = 0;
-        }
-    }
-    int order[2] = { 0, 1 };
-    if (cy == 2) {
-        const bool firstIsEven = (ey[0] & 1) == 0;
-        if (firstIsEven != evenRowsFirst) {
-            order[0] = 1;
-            order[1] = 0;
-        }
-    }
-    Adj a;
-    a.n = 0;
-    for (int r = 0; r < cy; ++r) {
-        const int rr = order[r];
-        for (int k = 0; k < cx; ++k) {
-            a.e[a.n] = ex[k] + nx * ey[rr];
-            a.loc[a.n] = lx[k] + (CG + 1) * ly[rr];
-            ++a.n;
-        }
-    }
-    return a;
 }
 
 } // namespace nsdg

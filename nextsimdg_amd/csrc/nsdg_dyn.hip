@@ -113,17 +113,16 @@ template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nn)
         return;
-    const Adj adj = adjacent<CG>(n % row, n / row, m.nx, m.ny, true);
     double acc = 0;
-    for (int k = 0; k < adj.n; ++k) {
+    forEachAdjacent<CG, true>(n % row, n / row, m.nx, m.ny, [&](int e, int loc) {
         double c[4][2], J[GG * GG];
-        elementCoords(m, adj.e[k], c);
+        elementCoords(m, e, c);
         jacobian<GG>(c, J);
         double me = 0;
         for (int q = 0; q < GG * GG; ++q)
-            me += Phi<CG, GG>::v[adj.loc[k]][q] * (J[q] * W<GG>::v[0][q]);
+            me += Phi<CG, GG>::v[loc][q] * (J[q] * W<GG>::v[0][q]);
         acc += me;
-    }
+    });
     (void)N;
     out[n] = acc;
 }
@@ -142,10 +141,8 @@ __global__ __launch_bounds__(kBlock) void k_dg2cg(int nx, int ny, int ne, const 
     if (n >= nn)
         return;
     const int nix = n % row, niy = n / row;
-    const Adj adj = adjacent<CG>(nix, niy, nx, ny, false);
     double a1 = 0, a2 = 0;
-    for (int k = 0; k < adj.n; ++k) {
-        const int e = adj.e[k], j = adj.loc[k];
+    forEachAdjacent<CG, false>(nix, niy, nx, ny, [&](int e, int j) {
         const int jx = j % (CG + 1), jy = j / (CG + 1);
         const double w = CG == 1 ? 0.25 : ((jx == 1 && jy == 1) ? 1.0 : ((jx == 1 || jy == 1) ? 0.5 : 0.25));
         double t1 = 0, t2 = 0;
@@ -158,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_dg2cg(int nx, int ny, int ne, const 
         }
         a1 += w * t1;
         a2 += w * t2;
-    }
+    });
     double f = 1.0;
     if (niy == 0 || niy == CG * ny)
         f *= 2.0;
@@ -221,10 +218,8 @@ __global__ void k_ssh_gradient(int nx, int ny, const double* cgS, const double* 
     if (n >= nn)
         return;
     const Ops<UNI> op { ops, stride };
-    const Adj adj = adjacent<1>(n % row, n / row, nx, ny, true);
     double u = 0, v = 0;
-    for (int k = 0; k < adj.n; ++k) {
-        const int e = adj.e[k], j = adj.loc[k];
+    forEachAdjacent<1, true>(n % row, n / row, nx, ny, [&](int e, int j) {
         const int id = (e / nx) * row + e % nx;
         const double l[4] = { cgS[id], cgS[id + 1], cgS[id + row], cgS[id + row + 1] };
         double tx = 0, ty = 0;
@@ -234,7 +229,7 @@ __global__ void k_ssh_gradient(int nx, int ny, const double* cgS, const double* 
         }
         u -= tx;
         v -= ty;
-    }
+    });
     gu[n] = u / lumped1[n];
     gv[n] = v / lumped1[n];
 }
@@ -307,6 +302,25 @@ struct ElemArgs {
     size_t stride;
     double Pstar, DM2, invAlpha, relax;
 };
+
+template <int S> __device__ __forceinline__ void loadTensor(const double* f, size_t ne, int e, double* a, double* b, double* c)
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        a[s] = f[(0 * S + s) * ne + e];
+        b[s] = f[(1 * S + s) * ne + e];
+        c[s] = f[(2 * S + s) * ne + e];
+    }
+}
+template <int S> __device__ __forceinline__ void storeTensor(double* f, size_t ne, int e, const double* a, const double* b, const double* c)
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        f[(0 * S + s) * ne + e] = a[s];
+        f[(1 * S + s) * ne + e] = b[s];
+        f[(2 * S + s) * ne + e] = c[s];
+    }
+}
 
 // projectVelocityToStrain for one element (CGDynamicsKernel.cpp:300-337)
 template <int CG, bool UNI>
@@ -392,30 +406,13 @@ __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const doub
             acc3[s] += b * r22;
         }
     }
+    // the previous stress is read only now, keeping its 3 S registers free above
+    loadTensor<S>(a.sig, ne, e, s11, s12, s22);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         s11[s] = s11[s] * a.relax + acc1[s];
         s12[s] = s12[s] * a.relax + acc2[s];
         s22[s] = s22[s] * a.relax + acc3[s];
-    }
-}
-
-template <int S> __device__ __forceinline__ void loadTensor(const double* f, size_t ne, int e, double* a, double* b, double* c)
-{
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        a[s] = f[(0 * S + s) * ne + e];
-        b[s] = f[(1 * S + s) * ne + e];
-        c[s] = f[(2 * S + s) * ne + e];
-    }
-}
-template <int S> __device__ __forceinline__ void storeTensor(double* f, size_t ne, int e, const double* a, const double* b, const double* c)
-{
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        f[(0 * S + s) * ne + e] = a[s];
-        f[(1 * S + s) * ne + e] = b[s];
-        f[(2 * S + s) * ne + e] = c[s];
     }
 }
 
@@ -431,7 +428,6 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
         strainLocal<CG, UNI>(a, e, e11, e12, e22);
     else
         loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
-    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
     stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
     storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
 }
@@ -453,7 +449,6 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
         return;
     double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
     loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
-    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
     stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
     storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
 }
@@ -561,16 +556,13 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_nod
             a.v[n] = 0.0;
             continue;
         }
-        const Adj adj = adjacent<CG>(nix, niy, a.nx, a.ny, true);
+        // land contributions are stored as exact zeros: subtracting them changes nothing
         double dx = 0.0, dy = 0.0;
-        for (int k = 0; k < adj.n; ++k) {
-            const int e = adj.e[k];
-            if (!a.ocean[e])
-                continue;
+        forEachAdjacent<CG, true>(nix, niy, a.nx, a.ny, [&](int e, int j) {
             const int le = (e % a.nx - ex0 + 1) + T::HX * (e / a.nx - ey0 + 1);
-            dx -= contrib[le][adj.loc[k]][0];
-            dy -= contrib[le][adj.loc[k]][1];
-        }
+            dx -= contrib[le][j][0];
+            dy -= contrib[le][j][1];
+        });
         momentumNode(a, n, dx, dy);
     }
 }
@@ -590,12 +582,10 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_divergen
     }
     const Ops<UNI> op { a.ops, a.stride };
     const int row = CG * a.nx + 1;
-    const Adj adj = adjacent<CG>(n % row, n / row, a.nx, a.ny, true);
     double dx = 0.0, dy = 0.0;
-    for (int k = 0; k < adj.n; ++k) {
-        const int e = adj.e[k], j = adj.loc[k];
+    forEachAdjacent<CG, true>(n % row, n / row, a.nx, a.ny, [&](int e, int j) {
         if (!a.ocean[e])
-            continue;
+            return;
         double s11[S], s12[S], s22[S];
         loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
         double p1 = 0, p2 = 0, p3 = 0, p4 = 0;
@@ -609,7 +599,7 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_divergen
         }
         dx -= p1 + p2;
         dy -= p3 + p4;
-    }
+    });
     a.dSx[n] = dx;
     a.dSy[n] = dy;
 }
