@@ -182,6 +182,28 @@ template <int D, int G> struct PsiEW {
     static constexpr Mat<G, D> t = make_psiew<D, G, 2>();
     static constexpr Mat<G, D> l = make_psiew<D, G, 3>();
 };
+// Projection onto the DG stress basis on an affine element: M^-1 (PSI diag(w J)) with the
+// element mass M = J diag(m), m_s = sum_q w_q PSI_s(q)^2 (the basis is orthogonal under the
+// Gauss rule), so iMJwPSI[s][q] = PSI_s(q) w_q / m_s, independent of the element.
+template <int S, int G> constexpr Mat<S, G * G> make_bproj()
+{
+    Mat<S, G * G> m {};
+    for (int s = 0; s < S; ++s) {
+        double ms = 0;
+        for (int q = 0; q < G * G; ++q) {
+            const double p = dg_basis(s, gauss_point(G, q % G), gauss_point(G, q / G));
+            ms += gauss_weight(G, q % G) * gauss_weight(G, q / G) * p * p;
+        }
+        for (int q = 0; q < G * G; ++q)
+            m.a[s][q] = dg_basis(s, gauss_point(G, q % G), gauss_point(G, q / G))
+                * (gauss_weight(G, q % G) * gauss_weight(G, q / G)) / ms;
+    }
+    return m;
+}
+template <int S, int G> struct BProj {
+    static constexpr Mat<S, G * G> v = make_bproj<S, G>();
+};
+
 template <int CG, int G> struct Phi {
     static constexpr Mat<cgdofs(CG), G * G> v = make_phi<CG, G, 0>();
     static constexpr Mat<cgdofs(CG), G * G> x = make_phi<CG, G, 1>();

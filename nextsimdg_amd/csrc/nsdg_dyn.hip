@@ -33,7 +33,7 @@ namespace nsdg {
 template <int CG> struct MomLayout {
     static constexpr int S = cg2dgstress(CG), G = gp1d(S), Q = G * G, N = cgdofs(CG);
     static constexpr int GX = 0, GY = S * N, B = 2 * S * N, D1 = 2 * S * N + S * Q, D2 = 3 * S * N + S * Q,
-                         SX = 4 * S * N + S * Q, SY = SX + 16, K = SY + 16;
+                         SX = 4 * S * N + S * Q, SY = SX + 16, QX = SY + 16, QY = QX + Q * N, K = QY + Q * N;
 };
 
 // InitializeDivSMatrices (ParametricMap.cpp:209-297), Cartesian.
@@ -102,6 +102,22 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
             put(L::B + s * Q + q, a);
         }
     }
+    // Gauss-point strain straight from the nodal velocities: PSI^T iMgradX/Y (Q x N)
+    for (int q = 0; q < Q; ++q)
+        for (int j = 0; j < N; ++j) {
+            double a = 0, b = 0;
+            for (int s = 0; s < S; ++s) {
+                double gx = 0, gy = 0;
+                for (int t = 0; t < S; ++t) {
+                    gx += Mi[s][t] * d1[j][t];
+                    gy += Mi[s][t] * d2[j][t];
+                }
+                a += Psi<S, G>::v[s][q] * gx;
+                b += Psi<S, G>::v[s][q] * gy;
+            }
+            put(L::QX + q * N + j, a);
+            put(L::QY + q * N + j, b);
+        }
 }
 
 // InitializeLumpedCGMassMatrix (ParametricMap.cpp:94-206): node-centric gather of the
@@ -301,6 +317,7 @@ struct ElemArgs {
     const double* ops;
     size_t stride;
     double Pstar, DM2, invAlpha, relax;
+    double xiX, etaX, xiY, etaY; // inverse map gradient of an affine uniform mesh
 };
 
 template <int S> __device__ __forceinline__ void loadTensor(const double* f, size_t ne, int e, double* a, double* b, double* c)
@@ -416,6 +433,154 @@ __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const doub
     }
 }
 
+// Uniform mesh (every element a translate of element 0, hence a parallelogram): every
+// element operator reduces to the reference-element basis, evaluated here from 1-D values
+// with sum factorisation (constexpr functions folded to literals, no operator loads).
+//  * Gauss-point strain: the exact gradient of the CG velocity (the DG stress space holds
+//    the CG derivatives, so the reference's L2 projection is exact), from PHI_x/PHI_y in
+//    tensor form and the four inverse-map scalars.
+//  * Projection back to the stress basis: M^-1 PSI diag(w J) = PSI w / m with
+//    m = (1, 1/12, 1/12, 1/180, 1/180, 1/144, 1/2160, 1/2160) on every affine element.
+// Land elements take their strain from the stored coefficients (the reference never
+// updates it).
+__device__ __forceinline__ constexpr double invMassRef(int s)
+{
+    return s == 0 ? 1.0 : (s <= 2 ? 12.0 : (s <= 4 ? 180.0 : (s == 5 ? 144.0 : 2160.0)));
+}
+// The 8 DG basis functions at (X, Y) = (x - 1/2, y - 1/2).
+__device__ __forceinline__ constexpr void dgBasisAt(double X, double Y, double* b)
+{
+    b[0] = 1.0;
+    b[1] = X;
+    b[2] = Y;
+    b[3] = X * X - 1.0 / 12.0;
+    b[4] = Y * Y - 1.0 / 12.0;
+    b[5] = X * Y;
+    b[6] = Y * (X * X - 1.0 / 12.0);
+    b[7] = X * (Y * Y - 1.0 / 12.0);
+}
+template <int CG, int DG>
+__device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
+    const size_t ne = a.ne;
+    const bool oc = a.ocean[e] != 0;
+    double buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
+    if (oc) {
+        const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+        const int cgi = CG * row * iy + CG * ix;
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                buf[c + K * r] = a.u[cgi + c + r * row];
+                buf[N + c + K * r] = a.v[cgi + c + r * row];
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3 * S; ++k)
+            buf[k] = a.eps[k * ne + e];
+    }
+    double hc[DG], ac[DG];
+#pragma unroll
+    for (int i = 0; i < DG; ++i) {
+        hc[i] = a.h[i * ne + e];
+        ac[i] = a.A[i * ne + e];
+    }
+    // previous stress, relaxed in place: the accumulators start at (1 - 1/alpha) sigma
+    double acc1[S], acc2[S], acc3[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        acc1[s] = a.sig[(0 * S + s) * ne + e] * a.relax;
+        acc2[s] = a.sig[(1 * S + s) * ne + e] * a.relax;
+        acc3[s] = a.sig[(2 * S + s) * ne + e] * a.relax;
+    }
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy) {
+        // velocity interpolated (Iu, Iv) and differentiated (Du, Dv) along y at this Gauss row
+        double Iu[K], Du[K], Iv[K], Dv[K];
+        if (oc) {
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                double iu = 0, du = 0, iv = 0, dv = 0;
+#pragma unroll
+                for (int jy = 0; jy < K; ++jy) {
+                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy));
+                    const double dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+                    iu += ph * buf[jx + K * jy];
+                    du += dp * buf[jx + K * jy];
+                    iv += ph * buf[N + jx + K * jy];
+                    dv += dp * buf[N + jx + K * jy];
+                }
+                Iu[jx] = iu;
+                Du[jx] = du;
+                Iv[jx] = iv;
+                Dv[jx] = dv;
+            }
+        }
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            double b[8];
+            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, b);
+            double g11 = 0, g12 = 0, g22 = 0;
+            if (oc) {
+                double uxi = 0, ueta = 0, vxi = 0, veta = 0;
+#pragma unroll
+                for (int jx = 0; jx < K; ++jx) {
+                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx));
+                    const double dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                    uxi += dp * Iu[jx];
+                    ueta += ph * Du[jx];
+                    vxi += dp * Iv[jx];
+                    veta += ph * Dv[jx];
+                }
+                g11 = uxi * a.xiX + ueta * a.etaX;
+                g22 = vxi * a.xiY + veta * a.etaY;
+                g12 = 0.5 * ((uxi * a.xiY + ueta * a.etaY) + (vxi * a.xiX + veta * a.etaX));
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    g11 += buf[s] * b[s];
+                    g12 += buf[S + s] * b[s];
+                    g22 += buf[2 * S + s] * b[s];
+                }
+            }
+            double hg = 0, ag = 0;
+#pragma unroll
+            for (int i = 0; i < DG; ++i) {
+                hg += hc[i] * b[i];
+                ag += ac[i] * b[i];
+            }
+            hg = fmax(hg, 0.0);
+            ag = fmin(fmax(ag, 0.0), 1.0);
+            const double X = a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12;
+            const double P = a.Pstar * hg * exp(-20.0 * (1.0 - ag));
+            const double c = (P * 0.125) * rsqrt(X); // P / (8 Delta)
+            const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.invAlpha;
+            const double r11 = w * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
+            const double r12 = w * ((2.0 * c) * g12);
+            const double r22 = w * (c * (5.0 * g22 + 3.0 * g11) - 0.5 * P);
+            // iMJwPSI[s][q] = PSI_s(q) w_q / m_s: w_q is in r, PSI_s(q) / m_s is a literal
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const double bs = b[s] * invMassRef(s);
+                if (bs == 0.0) // basis functions vanishing at this Gauss point (a literal test)
+                    continue;
+                acc1[s] += bs * r11;
+                acc2[s] += bs * r12;
+                acc3[s] += bs * r22;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        a.sig[(0 * S + s) * ne + e] = acc1[s];
+        a.sig[(1 * S + s) * ne + e] = acc2[s];
+        a.sig[(2 * S + s) * ne + e] = acc3[s];
+    }
+}
+
 // Fused strain + stress (one HBM pass per subcycle over the elements).
 template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_element(ElemArgs a)
 {
@@ -423,13 +588,17 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.ne)
         return;
-    double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
-    if (a.ocean[e])
-        strainLocal<CG, UNI>(a, e, e11, e12, e22);
-    else
-        loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
-    stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
-    storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    if constexpr (UNI) {
+        elementAffine<CG, DG>(a, e);
+    } else {
+        double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
+        if (a.ocean[e])
+            strainLocal<CG, UNI>(a, e, e11, e12, e22);
+        else
+            loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
+        stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
+        storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    }
 }
 template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_strain(ElemArgs a)
 {
@@ -469,24 +638,51 @@ struct NodeArgs {
     double rho_ice, F_atm, F_ocean, fc, gravity, beta, dt;
 };
 
-// updateMomentum for one node (VPCGDynamicsKernel.hpp:132-172), FP64.
+// updateMomentum for one node (VPCGDynamicsKernel.hpp:132-172), FP64: the node's inputs
+// are gathered first (NodeIn) so that a thread can keep several nodes' loads in flight.
+struct NodeIn {
+    double u, v, uo, vo, ua, va, H, A, ML, u0, v0, gx, gy;
+};
+__device__ __forceinline__ void loadNode(const NodeArgs& a, int n, NodeIn& in)
+{
+    in.u = a.u[n];
+    in.v = a.v[n];
+    in.uo = a.uo[n];
+    in.vo = a.vo[n];
+    in.ua = a.ua[n];
+    in.va = a.va[n];
+    in.H = a.H[n];
+    in.A = a.A[n];
+    in.ML = a.ML[n];
+    in.u0 = a.u0[n];
+    in.v0 = a.v0[n];
+    in.gx = a.gx[n];
+    in.gy = a.gy[n];
+}
+__device__ __forceinline__ void momentumUpdate(const NodeArgs& a, const NodeIn& in, double dSx, double dSy, double& un,
+    double& vn)
+{
+    const double uOcnRel = in.uo - in.u, vOcnRel = in.v - in.vo;
+    const double absatm = sqrt(in.ua * in.ua + in.va * in.va);
+    const double absocn = sqrt(uOcnRel * uOcnRel + vOcnRel * vOcnRel);
+    const double c1 = a.rho_ice * in.H / a.dt;
+    const double rhoH = a.rho_ice * in.H;
+    const double inv = 1.0 / (c1 * (1.0 + a.beta) + in.A * a.F_ocean * absocn);
+    un = inv
+        * (c1 * (a.beta * in.u + in.u0) + in.A * (a.F_atm * absatm * in.ua + a.F_ocean * absocn * in.uo)
+            - rhoH * a.fc * in.u - rhoH * a.gravity * in.gx + dSx / in.ML);
+    vn = inv
+        * (c1 * (a.beta * in.v + in.v0) + in.A * (a.F_atm * absatm * in.va + a.F_ocean * absocn * in.vo)
+            + rhoH * a.fc * in.v - rhoH * a.gravity * in.gy + dSy / in.ML);
+}
 __device__ __forceinline__ void momentumNode(const NodeArgs& a, int n, double dSx, double dSy)
 {
-    const double uu = a.u[n], vv = a.v[n];
-    const double uOcnRel = a.uo[n] - uu, vOcnRel = vv - a.vo[n];
-    const double ua = a.ua[n], va = a.va[n];
-    const double absatm = sqrt(ua * ua + va * va);
-    const double absocn = sqrt(uOcnRel * uOcnRel + vOcnRel * vOcnRel);
-    const double H = a.H[n], A = a.A[n], ML = a.ML[n];
-    const double c1 = a.rho_ice * H / a.dt;
-    const double rhoH = a.rho_ice * H;
-    const double inv = 1.0 / (c1 * (1.0 + a.beta) + A * a.F_ocean * absocn);
-    a.u[n] = inv
-        * (c1 * (a.beta * uu + a.u0[n]) + A * (a.F_atm * absatm * ua + a.F_ocean * absocn * a.uo[n])
-            - rhoH * a.fc * uu - rhoH * a.gravity * a.gx[n] + dSx / ML);
-    a.v[n] = inv
-        * (c1 * (a.beta * vv + a.v0[n]) + A * (a.F_atm * absatm * va + a.F_ocean * absocn * a.vo[n])
-            + rhoH * a.fc * vv - rhoH * a.gravity * a.gy[n] + dSy / ML);
+    NodeIn in;
+    loadNode(a, n, in);
+    double un, vn;
+    momentumUpdate(a, in, dSx, dSy, un, vn);
+    a.u[n] = un;
+    a.v[n] = vn;
 }
 
 template <int CG> struct NodeTile {
@@ -548,22 +744,41 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_nod
     const int nx1 = (ex0 + T::TX >= a.nx) ? CG * a.nx + 1 : CG * (ex0 + T::TX);
     const int ny1 = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 : CG * (ey0 + T::TY);
     const int wN = nx1 - nx0, cnt = wN * (ny1 - ny0);
-    for (int ln = threadIdx.x; ln < cnt; ln += kBlock) {
-        const int nix = nx0 + ln % wN, niy = ny0 + ln / wN;
-        const int n = nix + row * niy;
-        if (a.dirNode[n]) {
-            a.u[n] = 0.0;
-            a.v[n] = 0.0;
-            continue;
+    // NB nodes per thread per batch, all their loads issued before any arithmetic
+    constexpr int NB = 4;
+    for (int base = threadIdx.x; base < cnt; base += NB * kBlock) {
+        NodeIn in[NB];
+        int nid[NB], nxi[NB], nyi[NB];
+        bool live[NB], dir[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int ln = base + b * kBlock;
+            live[b] = ln < cnt;
+            nxi[b] = nx0 + ln % wN;
+            nyi[b] = ny0 + ln / wN;
+            nid[b] = nxi[b] + row * nyi[b];
+            dir[b] = live[b] ? a.dirNode[nid[b]] != 0 : true;
+            if (live[b] && !dir[b])
+                loadNode(a, nid[b], in[b]);
         }
-        // land contributions are stored as exact zeros: subtracting them changes nothing
-        double dx = 0.0, dy = 0.0;
-        forEachAdjacent<CG, true>(nix, niy, a.nx, a.ny, [&](int e, int j) {
-            const int le = (e % a.nx - ex0 + 1) + T::HX * (e / a.nx - ey0 + 1);
-            dx -= contrib[le][j][0];
-            dy -= contrib[le][j][1];
-        });
-        momentumNode(a, n, dx, dy);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (!live[b])
+                continue;
+            double un = 0.0, vn = 0.0;
+            if (!dir[b]) {
+                // land contributions are stored as exact zeros: subtracting them changes nothing
+                double dx = 0.0, dy = 0.0;
+                forEachAdjacent<CG, true>(nxi[b], nyi[b], a.nx, a.ny, [&](int e, int j) {
+                    const int le = (e % a.nx - ex0 + 1) + T::HX * (e / a.nx - ey0 + 1);
+                    dx -= contrib[le][j][0];
+                    dy -= contrib[le][j][1];
+                });
+                momentumUpdate(a, in[b], dx, dy, un, vn);
+            }
+            a.u[nid[b]] = un;
+            a.v[nid[b]] = vn;
+        }
     }
 }
 
@@ -1020,6 +1235,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.DM2 = p.DeltaMin * p.DeltaMin;
         a.invAlpha = 1.0 / p.alpha;
         a.relax = 1.0 - 1.0 / p.alpha;
+        a.xiX = mesh.xiX;
+        a.etaX = mesh.etaX;
+        a.xiY = mesh.xiY;
+        a.etaY = mesh.etaY;
         return a;
     }
     NodeArgs nodeArgs()

@@ -105,6 +105,8 @@ struct DeviceMesh {
     int nx = 0, ny = 0;
     size_t ne = 0;
     bool uniform = false;     // all elements congruent translates of element 0
+    bool affine = false;      // uniform, and element 0 is a parallelogram (constant Jacobian)
+    double xiX = 0, etaX = 0, xiY = 0, etaY = 0, detJ = 0; // inverse map gradient when affine
     bool hasPeriodic = false;
     DBuf<double> vxy;         // vertices, interleaved x, y
     DBuf<uint8_t> ocean;

@@ -130,6 +130,18 @@ void DeviceMesh::build(const HostMesh& m, hipStream_t s)
                     uniform = false;
                     break;
                 }
+    // Affine: element 0 is a parallelogram, so d(x,y)/d(xi,eta) is constant on it.
+    affine = uniform && std::fabs(off(0, 3, 0) - off(0, 1, 0) - off(0, 2, 0)) <= 1e-12 * h
+        && std::fabs(off(0, 3, 1) - off(0, 1, 1) - off(0, 2, 1)) <= 1e-12 * h;
+    if (affine) {
+        const double xXi = off(0, 1, 0), xEta = off(0, 2, 0), yXi = off(0, 1, 1), yEta = off(0, 2, 1);
+        detJ = xXi * yEta - xEta * yXi;
+        affine = detJ > 0;
+        xiX = yEta / detJ;
+        xiY = -xEta / detJ;
+        etaX = -yXi / detJ;
+        etaY = xXi / detJ;
+    }
 
     std::vector<uint32_t> fl(ne, 0);
     std::vector<double> fX(m.nx * (m.ny + 1), 1.0), fY((m.nx + 1) * m.ny, 1.0);
