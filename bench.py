@@ -25,13 +25,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def byte_model(nx, ny, cg=2, dg=6, we=8):
-    """Algorithmic HBM bytes of one subcycle (element pass, node pass)."""
-    S = 8 if cg == 2 else 3
+def byte_model(nx, ny, cg=2, we=8):
+    """Algorithmic HBM bytes of one subcycle (element pass, node pass): every value the
+    pass needs is moved once. Element pass: stress read + written (3 S coefficients each),
+    ice strength at the Q Gauss points, u and v at the nodes. Node pass: stress read; per
+    node u, v, the 7 frozen momentum coefficients read and u, v written."""
+    S, Q = (8, 9) if cg == 2 else (3, 4)
     ne = nx * ny
     nn = (cg * nx + 1) * (cg * ny + 1)
-    elem = we * (6 * S + 2 * dg) * ne + 8 * 2 * nn   # sigma r+w, h, A; u, v at nodes
-    node = we * 3 * S * ne + 8 * 15 * nn            # sigma r; 13 node fields r, u, v w
+    elem = we * (6 * S + Q) * ne + 8 * 2 * nn
+    node = we * 3 * S * ne + 8 * 11 * nn
     return elem, node
 
 
@@ -110,6 +113,20 @@ def cpu_baseline(nx, ny, seed=0, target_s=12.0):
     }
 
 
+def calibrate_copy(mib=1024, reps=3):
+    """A plain device-to-device copy of known size (torch, plumbing only), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs to bytes."""
+    import torch
+
+    n = mib * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 8, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,7 +136,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB device copy (for counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design "
               "note in DESIGN.md, not a measured path", file=sys.stderr)

@@ -178,7 +178,7 @@ __device__ __forceinline__ void axisCandidates(int cg, int ni, int n, int& c0, i
     h1 = c < n;
 }
 template <int CG, bool EVEN_ROWS_FIRST, class F>
-__device__ __forceinline__ void forEachAdjacent(int nix, int niy, int nx, int ny, F&& f)
+__device__ __forceinline__ void forEachAdjacentXY(int nix, int niy, int nx, int ny, F&& f)
 {
     int xa, la, xb, lb, ya, ma, yb, mb;
     bool ha, hb, ga, gb;
@@ -203,9 +203,28 @@ __device__ __forceinline__ void forEachAdjacent(int nix, int niy, int nx, int ny
             const int ex = c == 0 ? xa : xb, mx = c == 0 ? la : lb;
             if (!hc)
                 continue;
-            f(ex + nx * ey, mx + (CG + 1) * my);
+            f(ex, ey, mx + (CG + 1) * my);
         }
     }
 }
+template <int CG, bool EVEN_ROWS_FIRST, class F>
+__device__ __forceinline__ void forEachAdjacent(int nix, int niy, int nx, int ny, F&& f)
+{
+    forEachAdjacentXY<CG, EVEN_ROWS_FIRST>(nix, niy, nx, ny, [&](int ex, int ey, int loc) { f(ex + nx * ey, loc); });
+}
+
+// Block -> tile map that keeps consecutive tiles on one XCD: workgroups are dispatched
+// round-robin over the 8 XCDs, so block b runs on XCD b % 8; giving that XCD the
+// contiguous tile range [xcd * chunk, (xcd + 1) * chunk) keeps a tile's left and
+// lower neighbours (its halo) in the same L2. The grid is padded to 8 * chunk blocks;
+// returns -1 for the padding.
+constexpr int kXCD = 8;
+__device__ __forceinline__ int xcdTile(int b, int ntiles)
+{
+    const int chunk = (ntiles + kXCD - 1) / kXCD;
+    const int t = (b % kXCD) * chunk + b / kXCD;
+    return t < ntiles ? t : -1;
+}
+inline unsigned xcdGrid(int ntiles) { return static_cast<unsigned>(kXCD * ((ntiles + kXCD - 1) / kXCD)); }
 
 } // namespace nsdg

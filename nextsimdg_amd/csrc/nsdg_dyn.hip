@@ -316,6 +316,7 @@ struct ElemArgs {
     double* sig; // SoA [3][S][ne]
     const double* ops;
     size_t stride;
+    const double* Pg; // SoA [Q][ne] ice strength at the Gauss points (fixed over the subcycles)
     double Pstar, DM2, invAlpha, relax;
     double xiX, etaX, xiY, etaY; // inverse map gradient of an affine uniform mesh
 };
@@ -373,9 +374,60 @@ __device__ __forceinline__ void strainLocal(const ElemArgs& a, int e, double* e1
     }
 }
 
+// The 8 DG basis functions at (X, Y) = (x - 1/2, y - 1/2).
+__device__ __forceinline__ constexpr void dgBasisAt(double X, double Y, double* b)
+{
+    b[0] = 1.0;
+    b[1] = X;
+    b[2] = Y;
+    b[3] = X * X - 1.0 / 12.0;
+    b[4] = Y * Y - 1.0 / 12.0;
+    b[5] = X * Y;
+    b[6] = Y * (X * X - 1.0 / 12.0);
+    b[7] = X * (Y * Y - 1.0 / 12.0);
+}
+// Ice strength at a Gauss point from the interpolated h and A, clipped to h >= 0 and
+// 0 <= A <= 1: P = P* h exp(-20 (1 - A)) (MEVPStressUpdateStep.hpp:64-72).
+__device__ __forceinline__ double iceStrength(double Pstar, double hg, double ag)
+{
+    hg = fmax(hg, 0.0);
+    ag = fmin(fmax(ag, 0.0), 1.0);
+    return Pstar * hg * exp(-20.0 * (1.0 - ag));
+}
+// The strength depends only on h and A, which the subcycles do not change: evaluated once
+// per timestep at the Gauss points of every element, Pg[q * ne + e], q = gx + G gy.
+template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_strength(int ne, const double* h, const double* A,
+    double Pstar, double* Pg)
+{
+    constexpr int NG = MomLayout<CG>::G;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    double hc[DG], ac[DG];
+#pragma unroll
+    for (int i = 0; i < DG; ++i) {
+        hc[i] = h[static_cast<size_t>(i) * ne + e];
+        ac[i] = A[static_cast<size_t>(i) * ne + e];
+    }
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy)
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            double b[8];
+            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, b);
+            double hg = 0, ag = 0;
+#pragma unroll
+            for (int i = 0; i < DG; ++i) {
+                hg += hc[i] * b[i];
+                ag += ac[i] * b[i];
+            }
+            Pg[static_cast<size_t>(gx + NG * gy) * ne + e] = iceStrength(Pstar, hg, ag);
+        }
+}
+
 // stressUpdateHighOrder for one element (MEVPStressUpdateStep.hpp:30-118): sigma <-
 // (1 - 1/alpha) sigma + iMJwPSI (1/alpha) sigma_VP(strain, h, A) at the Gauss points.
-template <int CG, int DG, bool UNI>
+template <int CG, int DG, bool UNI, bool PG>
 __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const double* e11, const double* e12,
     const double* e22, double* s11, double* s12, double* s22)
 {
@@ -384,10 +436,12 @@ __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const doub
     const Ops<UNI> op { a.ops, a.stride };
     const size_t ne = a.ne;
     double hc[DG], ac[DG];
+    if constexpr (!PG) {
 #pragma unroll
-    for (int i = 0; i < DG; ++i) {
-        hc[i] = a.h[i * ne + e];
-        ac[i] = a.A[i * ne + e];
+        for (int i = 0; i < DG; ++i) {
+            hc[i] = a.h[i * ne + e];
+            ac[i] = a.A[i * ne + e];
+        }
     }
     double acc1[S], acc2[S], acc3[S];
 #pragma unroll
@@ -395,14 +449,7 @@ __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const doub
         acc1[s] = acc2[s] = acc3[s] = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        double hg = 0, ag = 0, g11 = 0, g12 = 0, g22 = 0;
-#pragma unroll
-        for (int i = 0; i < DG; ++i) {
-            hg += hc[i] * Psi<DG, NG>::v[i][q];
-            ag += ac[i] * Psi<DG, NG>::v[i][q];
-        }
-        hg = fmax(hg, 0.0);
-        ag = fmin(fmax(ag, 0.0), 1.0);
+        double g11 = 0, g12 = 0, g22 = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             g11 += e11[s] * Psi<S, NG>::v[s][q];
@@ -410,7 +457,18 @@ __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const doub
             g22 += e22[s] * Psi<S, NG>::v[s][q];
         }
         const double delta = sqrt(a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12);
-        const double P = a.Pstar * hg * exp(-20.0 * (1.0 - ag));
+        double P;
+        if constexpr (PG) {
+            P = a.Pg[q * ne + e];
+        } else {
+            double hg = 0, ag = 0;
+#pragma unroll
+            for (int i = 0; i < DG; ++i) {
+                hg += hc[i] * Psi<DG, NG>::v[i][q];
+                ag += ac[i] * Psi<DG, NG>::v[i][q];
+            }
+            P = iceStrength(a.Pstar, hg, ag);
+        }
         const double c = (P * 0.125) / delta; // P/8/Delta
         const double r11 = a.invAlpha * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
         const double r12 = a.invAlpha * ((2.0 * c) * g12);
@@ -447,18 +505,6 @@ __device__ __forceinline__ constexpr double invMassRef(int s)
 {
     return s == 0 ? 1.0 : (s <= 2 ? 12.0 : (s <= 4 ? 180.0 : (s == 5 ? 144.0 : 2160.0)));
 }
-// The 8 DG basis functions at (X, Y) = (x - 1/2, y - 1/2).
-__device__ __forceinline__ constexpr void dgBasisAt(double X, double Y, double* b)
-{
-    b[0] = 1.0;
-    b[1] = X;
-    b[2] = Y;
-    b[3] = X * X - 1.0 / 12.0;
-    b[4] = Y * Y - 1.0 / 12.0;
-    b[5] = X * Y;
-    b[6] = Y * (X * X - 1.0 / 12.0);
-    b[7] = X * (Y * Y - 1.0 / 12.0);
-}
 template <int CG, int DG>
 __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
 {
@@ -481,12 +527,6 @@ __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
 #pragma unroll
         for (int k = 0; k < 3 * S; ++k)
             buf[k] = a.eps[k * ne + e];
-    }
-    double hc[DG], ac[DG];
-#pragma unroll
-    for (int i = 0; i < DG; ++i) {
-        hc[i] = a.h[i * ne + e];
-        ac[i] = a.A[i * ne + e];
     }
     // previous stress, relaxed in place: the accumulators start at (1 - 1/alpha) sigma
     double acc1[S], acc2[S], acc3[S];
@@ -546,16 +586,8 @@ __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
                     g22 += buf[2 * S + s] * b[s];
                 }
             }
-            double hg = 0, ag = 0;
-#pragma unroll
-            for (int i = 0; i < DG; ++i) {
-                hg += hc[i] * b[i];
-                ag += ac[i] * b[i];
-            }
-            hg = fmax(hg, 0.0);
-            ag = fmin(fmax(ag, 0.0), 1.0);
             const double X = a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12;
-            const double P = a.Pstar * hg * exp(-20.0 * (1.0 - ag));
+            const double P = a.Pg[(gx + NG * gy) * ne + e];
             const double c = (P * 0.125) * rsqrt(X); // P / (8 Delta)
             const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.invAlpha;
             const double r11 = w * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
@@ -596,7 +628,7 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
             strainLocal<CG, UNI>(a, e, e11, e12, e22);
         else
             loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
-        stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
+        stressLocal<CG, DG, UNI, true>(a, e, e11, e12, e22, s11, s12, s22);
         storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
     }
 }
@@ -618,7 +650,7 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
         return;
     double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
     loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
-    stressLocal<CG, DG, UNI>(a, e, e11, e12, e22, s11, s12, s22);
+    stressLocal<CG, DG, UNI, false>(a, e, e11, e12, e22, s11, s12, s22);
     storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
 }
 
@@ -633,9 +665,14 @@ struct NodeArgs {
     double* u;
     double* v;
     const double *u0, *v0, *uo, *vo, *ua, *va, *H, *A, *gx, *gy, *ML;
+    const double2* ocn; // frozen per-node coefficients, see k_node_coeffs
+    const double2* cf;
+    const double2* qf;
+    const double* iML;
     double* dSx;
     double* dSy;
     double rho_ice, F_atm, F_ocean, fc, gravity, beta, dt;
+    double xiX, etaX, xiY, etaY, detJ; // affine uniform mesh geometry
 };
 
 // updateMomentum for one node (VPCGDynamicsKernel.hpp:132-172), FP64: the node's inputs
@@ -685,22 +722,154 @@ __device__ __forceinline__ void momentumNode(const NodeArgs& a, int n, double dS
     a.v[n] = vn;
 }
 
+// Weak-divergence contributions of one element to its CG nodes:
+// tx_j = (divS1 s11 + divS2 s12)_j, ty_j = (divS1 s12 + divS2 s22)_j (CGDynamicsKernel.cpp:339-363).
+template <int CG, bool UNI>
+__device__ __forceinline__ void divergenceLocal(const NodeArgs& a, int e, double* tx, double* ty)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, N = cgdofs(CG);
+    double s11[S], s12[S], s22[S];
+    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    const Ops<UNI> op { a.ops, a.stride };
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double p1 = 0, p2 = 0, p3 = 0, p4 = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double d1 = op(L::D1 + j * S + s, e), d2 = op(L::D2 + j * S + s, e);
+            p1 += d1 * s11[s];
+            p2 += d2 * s12[s];
+            p3 += d1 * s12[s];
+            p4 += d2 * s22[s];
+        }
+        tx[j] = p1 + p2;
+        ty[j] = p3 + p4;
+    }
+}
+// The same on a uniform (affine) mesh with literal reference-element values and sum
+// factorisation: stress at the Gauss points, the Jacobian-weighted fluxes
+// T = w |J| J^-T-rows applied to (s11, s12) and (s12, s22), then
+// tx_j = sum_q (PHI_xi[j](q) Tx_xi(q) + PHI_eta[j](q) Tx_eta(q)), factorised per Gauss row.
+template <int CG>
+__device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, double* tx, double* ty)
+{
+    constexpr int S = cg2dgstress(CG), NG = gp1d(S), N = cgdofs(CG), K = CG + 1;
+    double s11[S], s12[S], s22[S];
+    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        tx[j] = ty[j] = 0.0;
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy) {
+        double ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
+#pragma unroll
+        for (int jx = 0; jx < K; ++jx)
+            ax[jx] = bx[jx] = ay[jx] = by[jx] = 0.0;
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            double b[8];
+            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, b);
+            double v11 = 0, v12 = 0, v22 = 0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                v11 += s11[s] * b[s];
+                v12 += s12[s] * b[s];
+                v22 += s22[s] * b[s];
+            }
+            const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.detJ;
+            // grad phi = (phi_xi xiX + phi_eta etaX, phi_xi xiY + phi_eta etaY)
+            const double fxXi = w * (a.xiX * v11 + a.xiY * v12), fxEta = w * (a.etaX * v11 + a.etaY * v12);
+            const double fyXi = w * (a.xiX * v12 + a.xiY * v22), fyEta = w * (a.etaX * v12 + a.etaY * v22);
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                const double ph = cg_1d(CG, jx, gauss_point(NG, gx));
+                const double dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                ax[jx] += dp * fxXi;  // d/dxi part, integrated along x
+                bx[jx] += ph * fxEta; // d/deta part
+                ay[jx] += dp * fyXi;
+                by[jx] += ph * fyEta;
+            }
+        }
+#pragma unroll
+        for (int jy = 0; jy < K; ++jy) {
+            const double ph = cg_1d(CG, jy, gauss_point(NG, gy));
+            const double dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                tx[jx + K * jy] += ph * ax[jx] + dp * bx[jx];
+                ty[jx + K * jy] += ph * ay[jx] + dp * by[jx];
+            }
+        }
+    }
+}
+
+// The momentum update with the terms that stay fixed over the subcycles of one timestep
+// (u0, v0, wind, H, A, SSH gradient, lumped mass) folded once per timestep into
+//   ocn = (uo, vo), cf = (c1, A F_ocean) with c1 = rho_ice H / dt, iML = 1 / ML,
+//   qf = (c1 u0 + A F_atm |ua| ua - rho_ice H g gx, c1 v0 + A F_atm |ua| va - rho_ice H g gy),
+// so that a subcycle reads 9 values per node instead of 13; the update itself is the one
+// of VPCGDynamicsKernel.hpp:132-172, same-component Coriolis term included.
+__global__ __launch_bounds__(kBlock) void k_node_coeffs(NodeArgs a, double2* ocn, double2* cf, double2* qf, double* iML)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nn)
+        return;
+    const double H = a.H[n], A = a.A[n], ua = a.ua[n], va = a.va[n];
+    const double absatm = sqrt(ua * ua + va * va);
+    const double c1 = a.rho_ice * H / a.dt, rhoH = a.rho_ice * H;
+    ocn[n] = make_double2(a.uo[n], a.vo[n]);
+    cf[n] = make_double2(c1, A * a.F_ocean);
+    qf[n] = make_double2(c1 * a.u0[n] + A * (a.F_atm * absatm * ua) - rhoH * a.gravity * a.gx[n],
+        c1 * a.v0[n] + A * (a.F_atm * absatm * va) - rhoH * a.gravity * a.gy[n]);
+    iML[n] = 1.0 / a.ML[n];
+}
+struct NodeFz {
+    double u, v, iML;
+    double2 ocn, cf, qf;
+};
+__device__ __forceinline__ void loadFrozen(const NodeArgs& a, int n, NodeFz& in)
+{
+    in.u = a.u[n];
+    in.v = a.v[n];
+    in.ocn = a.ocn[n];
+    in.cf = a.cf[n];
+    in.qf = a.qf[n];
+    in.iML = a.iML[n];
+}
+__device__ __forceinline__ void momentumFrozen(const NodeArgs& a, const NodeFz& in, double dSx, double dSy, double& un,
+    double& vn)
+{
+    const double uOcnRel = in.ocn.x - in.u, vOcnRel = in.v - in.ocn.y;
+    const double absocn = sqrt(uOcnRel * uOcnRel + vOcnRel * vOcnRel);
+    const double c1 = in.cf.x, drag = in.cf.y * absocn;
+    const double cor = c1 * a.dt * a.fc; // rho_ice H fc
+    const double inv = 1.0 / (c1 * (1.0 + a.beta) + drag);
+    un = inv * (c1 * a.beta * in.u + in.qf.x + drag * in.ocn.x - cor * in.u + dSx * in.iML);
+    vn = inv * (c1 * a.beta * in.v + in.qf.y + drag * in.ocn.y + cor * in.v + dSy * in.iML);
+}
+
 template <int CG> struct NodeTile {
-    static constexpr int TX = CG == 2 ? 32 : 16, TY = CG == 2 ? 8 : 16;
+    // 64 x 4 elements: long rows keep the 128-byte lines of the stress and node arrays
+    // well used; the tile plus its left/lower halo is 65 x 5 elements.
+    static constexpr int TX = 64, TY = 4;
     static constexpr int HX = TX + 1, HY = TY + 1, N = cgdofs(CG);
+    static constexpr int CW = CG * TX;                        // owned node columns
+    static constexpr int TPR = CW < kBlock ? CW : kBlock;     // threads per node row
+    static constexpr int RS = kBlock / TPR;                   // node rows per block sweep
 };
 
 // Fused stressDivergence + updateMomentum + applyBoundaries (no periodic segments).
 template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
 {
     using T = NodeTile<CG>;
-    using L = MomLayout<CG>;
-    constexpr int S = L::S, N = T::N;
+    constexpr int N = T::N;
     __shared__ double contrib[T::HX * T::HY][N][2];
-    const int tilesX = (a.nx + T::TX - 1) / T::TX;
-    const int ex0 = (blockIdx.x % tilesX) * T::TX, ey0 = (blockIdx.x / tilesX) * T::TY;
-    const Ops<UNI> op { a.ops, a.stride };
-    const size_t ne = a.ne;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
 
     // Phase 1: weak-divergence contributions of the tile and its left/bottom halo.
     for (int le = threadIdx.x; le < T::HX * T::HY; le += kBlock) {
@@ -712,22 +881,10 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_nod
         if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
             const int e = ex + a.nx * ey;
             if (a.ocean[e]) {
-                double s11[S], s12[S], s22[S];
-                loadTensor<S>(a.sig, ne, e, s11, s12, s22);
-#pragma unroll
-                for (int j = 0; j < N; ++j) {
-                    double p1 = 0, p2 = 0, p3 = 0, p4 = 0;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const double d1 = op(L::D1 + j * S + s, e), d2 = op(L::D2 + j * S + s, e);
-                        p1 += d1 * s11[s];
-                        p2 += d2 * s12[s];
-                        p3 += d1 * s12[s];
-                        p4 += d2 * s22[s];
-                    }
-                    tx[j] = p1 + p2;
-                    ty[j] = p3 + p4;
-                }
+                if constexpr (UNI)
+                    divergenceAffine<CG>(a, e, tx, ty);
+                else
+                    divergenceLocal<CG, false>(a, e, tx, ty);
             }
         }
 #pragma unroll
@@ -738,46 +895,46 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_nod
     }
     __syncthreads();
 
-    // Phase 2: nodes owned by the tile (the last tile row/column also owns the domain edge).
+    // Phase 2: nodes owned by the tile (the last tile row/column also owns the domain edge);
+    // each thread updates up to NB nodes of one column, all their loads issued first.
     const int row = CG * a.nx + 1;
     const int nx0 = CG * ex0, ny0 = CG * ey0;
-    const int nx1 = (ex0 + T::TX >= a.nx) ? CG * a.nx + 1 : CG * (ex0 + T::TX);
-    const int ny1 = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 : CG * (ey0 + T::TY);
-    const int wN = nx1 - nx0, cnt = wN * (ny1 - ny0);
-    // NB nodes per thread per batch, all their loads issued before any arithmetic
-    constexpr int NB = 4;
-    for (int base = threadIdx.x; base < cnt; base += NB * kBlock) {
-        NodeIn in[NB];
-        int nid[NB], nxi[NB], nyi[NB];
-        bool live[NB], dir[NB];
+    const int W = (ex0 + T::TX >= a.nx) ? CG * a.nx + 1 - nx0 : T::CW;
+    const int H = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 - ny0 : CG * T::TY;
+    constexpr int NB = CG * T::TY / T::RS;
+    for (int c = threadIdx.x % T::TPR; c < W; c += T::TPR) {
+        for (int r0 = threadIdx.x / T::TPR; r0 < H; r0 += NB * T::RS) {
+            NodeFz in[NB];
+            bool live[NB], dir[NB];
+            const int nix = nx0 + c;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int ln = base + b * kBlock;
-            live[b] = ln < cnt;
-            nxi[b] = nx0 + ln % wN;
-            nyi[b] = ny0 + ln / wN;
-            nid[b] = nxi[b] + row * nyi[b];
-            dir[b] = live[b] ? a.dirNode[nid[b]] != 0 : true;
-            if (live[b] && !dir[b])
-                loadNode(a, nid[b], in[b]);
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (!live[b])
-                continue;
-            double un = 0.0, vn = 0.0;
-            if (!dir[b]) {
-                // land contributions are stored as exact zeros: subtracting them changes nothing
-                double dx = 0.0, dy = 0.0;
-                forEachAdjacent<CG, true>(nxi[b], nyi[b], a.nx, a.ny, [&](int e, int j) {
-                    const int le = (e % a.nx - ex0 + 1) + T::HX * (e / a.nx - ey0 + 1);
-                    dx -= contrib[le][j][0];
-                    dy -= contrib[le][j][1];
-                });
-                momentumUpdate(a, in[b], dx, dy, un, vn);
+            for (int b = 0; b < NB; ++b) {
+                const int r = r0 + b * T::RS;
+                live[b] = r < H;
+                const int n = nix + row * (ny0 + r);
+                dir[b] = live[b] ? a.dirNode[n] != 0 : true;
+                if (live[b] && !dir[b])
+                    loadFrozen(a, n, in[b]);
             }
-            a.u[nid[b]] = un;
-            a.v[nid[b]] = vn;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (!live[b])
+                    continue;
+                const int niy = ny0 + r0 + b * T::RS;
+                double un = 0.0, vn = 0.0;
+                if (!dir[b]) {
+                    // land contributions are stored as exact zeros: subtracting them changes nothing
+                    double dx = 0.0, dy = 0.0;
+                    forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
+                        const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
+                        dx -= contrib[le][j][0];
+                        dy -= contrib[le][j][1];
+                    });
+                    momentumFrozen(a, in[b], dx, dy, un, vn);
+                }
+                a.u[nix + row * niy] = un;
+                a.v[nix + row * niy] = vn;
+            }
         }
     }
 }
@@ -952,6 +1109,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<double> u, v, u0, v0, uo, vo, ua, va, cgH, cgA, gX, gY, dSx, dSy;
     DBuf<double> tmpDG, tmpDG2, tmpCG, tmpCG2, cg1a, cg1b, cg1c, scratch, dg0a, dg0b, part, res;
     DBuf<uint8_t> dirNode;
+    DBuf<double2> fzOcn, fzCf, fzQ; // frozen momentum coefficients (k_node_coeffs)
+    DBuf<double> fzIML, strength;   // 1 / ML; ice strength at the Gauss points
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
 
     explicit DynImpl(const nsdg_params& prm)
@@ -1002,6 +1161,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             b->alloc(nn);
         for (auto* b : { &cg1a, &cg1b, &cg1c })
             b->alloc(nn1);
+        for (auto* b : { &fzOcn, &fzCf, &fzQ })
+            b->alloc(nn);
+        fzIML.alloc(nn);
+        strength.alloc(static_cast<size_t>(L::Q) * ne);
         part.alloc(4 * blocksFor(ne));
         res.alloc(2);
         for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
@@ -1231,6 +1394,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.sig = sig.p;
         a.ops = mops.p;
         a.stride = opStride();
+        a.Pg = strength.p;
         a.Pstar = p.Pstar;
         a.DM2 = p.DeltaMin * p.DeltaMin;
         a.invAlpha = 1.0 / p.alpha;
@@ -1266,6 +1430,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.gx = gX.p;
         a.gy = gY.p;
         a.ML = lumped.p;
+        a.ocn = fzOcn.p;
+        a.cf = fzCf.p;
+        a.qf = fzQ.p;
+        a.iML = fzIML.p;
         a.dSx = dSx.p;
         a.dSy = dSy.p;
         a.rho_ice = p.rho_ice;
@@ -1275,6 +1443,11 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.gravity = p.gravity;
         a.beta = p.beta;
         a.dt = deltaT;
+        a.xiX = mesh.xiX;
+        a.etaX = mesh.etaX;
+        a.xiY = mesh.xiY;
+        a.etaY = mesh.etaY;
+        a.detJ = mesh.detJ;
         return a;
     }
     void elementPass()
@@ -1293,11 +1466,11 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         if (!mesh.hasPeriodic) {
             Timed t(prof, s, "mevp_node");
             using T = NodeTile<CG>;
-            const unsigned tiles = static_cast<unsigned>(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+            const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
             if (mesh.uniform)
-                k_mevp_node<CG, true><<<tiles, kBlock, 0, s>>>(a);
+                k_mevp_node<CG, true><<<grid, kBlock, 0, s>>>(a);
             else
-                k_mevp_node<CG, false><<<tiles, kBlock, 0, s>>>(a);
+                k_mevp_node<CG, false><<<grid, kBlock, 0, s>>>(a);
             checkLaunch("mevp_node");
             return;
         }
@@ -1337,8 +1510,20 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         deviceCopy(v0.p, v.p, nn, s);
         deltaT = dt;
     }
+    // Terms fixed over the subcycles, evaluated from the current fields at the start of
+    // every run of subcycles (so set_field between runs is honoured).
+    void frozenAsync()
+    {
+        Timed t(prof, s, "frozen_terms");
+        k_strength<CG, DG><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar, strength.p);
+        checkLaunch("strength");
+        k_node_coeffs<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), fzOcn.p, fzCf.p, fzQ.p, fzIML.p);
+        checkLaunch("node_coeffs");
+    }
     void subcyclesAsync(int n)
     {
+        if (n > 0)
+            frozenAsync();
         for (int k = 0; k < n; ++k) {
             elementPass();
             nodePass();
