@@ -38,14 +38,16 @@ def byte_model(nx, ny, cg=2, we=8):
     return elem, node
 
 
-def make_kernel(nx, ny, seed, prec):
+def make_kernel(nx, ny, seed, prec, kind="plain", perturb=0.0):
+    import numpy as np
     import nextsimdg_amd as nd
     from nextsimdg_amd import inputs
 
-    s = inputs.benchmark_setup(nx, ny, seed)
+    s = inputs.benchmark_setup(nx, ny, seed, kind=kind)
     k = nd.MEVPDynamicsKernel(2, 6)
     k.initialise(s["coords"], False, s["mask"], nx, ny)
-    k.setData("hice", s["h"], 1)
+    h = s["h"] * (1.0 + perturb * np.random.default_rng(seed + 777).standard_normal(s["h"].size))
+    k.setData("hice", h, 1)
     k.setData("cice", s["A"], 1)
     k.setData("uocean", s["uocean"], 1)
     k.setData("vocean", s["vocean"], 1)
@@ -63,8 +65,8 @@ def run_steps(k, off, L, t0, n, dt):
         k.update(dt)
 
 
-def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0):
-    k, off, L = make_kernel(nx, ny, seed, prec)
+def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0, kind="plain"):
+    k, off, L = make_kernel(nx, ny, seed, prec, kind)
     run_steps(k, off, L, 0, warmup, dt)
     k.synchronize()
     t = time.perf_counter()
@@ -78,6 +80,44 @@ def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0):
     stats = {n: k.kernel_stats(n) for n in ("mevp_element", "mevp_node", "transport_op", "dg2cg", "cg2dg")}
     k.set_profiling(False)
     return el, stats
+
+
+def fp32_variant(nx, steps, warmup, err_steps, seed=0):
+    """The FP32 variant on heterogeneous ice (h = 0.3 + LogNormal(0, 0.3), A ~ Beta(8, 1)):
+    throughput next to the FP64 path on the same inputs, and the per-field relative L2
+    error of FP32 against FP64 after err_steps model steps, beside the conditioning floor:
+    the same FP64 run with h perturbed by a relative 1e-7 (one FP32 rounding)."""
+    import numpy as np
+
+    kind = "heterogeneous"
+    t64, _ = measure(nx, nx, steps, warmup, seed=seed, prec=0, kind=kind)
+    t32, st32 = measure(nx, nx, steps, warmup, seed=seed, prec=1, kind=kind)
+    fields = {}
+    for prec, pert in ((0, 0.0), (1, 0.0), ("floor", 1e-7)):
+        k, off, L = make_kernel(nx, nx, seed, 0 if prec == "floor" else prec, kind, pert)
+        run_steps(k, off, L, 0, err_steps, 120.0)
+        k.synchronize()
+        f = {n: k.get_cg(n) for n in ("u", "v")}
+        f.update({n: k.getDGData(n, 6) for n in ("hice", "cice")})
+        f.update({n: k.getDGData(n, 8) for n in ("s11", "s12", "s22")})
+        fields[prec] = f
+        del k
+    def rel(a):
+        return {n: float(f"{np.linalg.norm(fields[a][n] - fields[0][n]) / np.linalg.norm(fields[0][n]):.3e}")
+                for n in fields[0]}
+    iters = nx * nx * 100 * steps
+    return {
+        "workload": f"mEVP dynamics step, {nx}x{nx} CG2/DG2, heterogeneous ice, FP32 stress/strength storage "
+                    "and element arithmetic (Kahan-compensated integrals, stochastic rounding of the stored "
+                    "stress), FP64 velocity update",
+        "value": round(iters / t32 / 1e6, 3),
+        "fp64_value": round(iters / t64 / 1e6, 3),
+        "speedup_vs_fp64": round(t64 / t32, 3),
+        "kernels_ms": {n: (round(t / c, 5) if c else None) for n, (c, t) in st32.items()},
+        "rel_l2_vs_fp64": rel(1),
+        "fp64_rel_l2_under_1e-7_input_perturbation": rel("floor"),
+        "error_after_steps": err_steps,
+    }
 
 
 def cpu_baseline(nx, ny, seed=0, target_s=12.0):
@@ -134,6 +174,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp32-nx", type=int, default=1024, help="FP32 variant size (0 skips it)")
+    ap.add_argument("--fp32-error-steps", type=int, default=100)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -186,6 +228,9 @@ def main():
         "subcycle_hbm_frac": round((eb + nb) / ((stats["mevp_element"][1] + stats["mevp_node"][1])
                                                 / stats["mevp_element"][0] * 1e-3) / HBM_PEAK, 4),
     }
+    if args.fp32_nx:
+        log(f"bench: FP32 variant {args.fp32_nx}^2 ...")
+        out["fp32"] = fp32_variant(args.fp32_nx, max(2, args.steps // 2), 1, args.fp32_error_steps)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:

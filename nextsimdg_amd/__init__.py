@@ -9,4 +9,5 @@ synthetic benchmark inputs. There is no CPU fallback.
 from .dynamics import (  # noqa: F401
     PREC_FP32, PREC_FP64, DGTransport, LimitMax, LimitMin, MEVPDynamicsKernel, ParametricMesh, default_params,
 )
+from ._lib import NsdgError  # noqa: F401
 from . import inputs  # noqa: F401

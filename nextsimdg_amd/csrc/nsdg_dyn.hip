@@ -317,11 +317,14 @@ struct ElemArgs {
     const double* ops;
     size_t stride;
     const double* Pg; // SoA [Q][ne] ice strength at the Gauss points (fixed over the subcycles)
+    float* sigF;      // FP32 variant: stress and strength in single precision
+    const float* PgF;
+    uint32_t iter;    // subcycle counter (random bits of the FP32 stochastic rounding)
     double Pstar, DM2, invAlpha, relax;
     double xiX, etaX, xiY, etaY; // inverse map gradient of an affine uniform mesh
 };
 
-template <int S> __device__ __forceinline__ void loadTensor(const double* f, size_t ne, int e, double* a, double* b, double* c)
+template <int S, class T, class U> __device__ __forceinline__ void loadTensor(const T* f, size_t ne, int e, U* a, U* b, U* c)
 {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -396,8 +399,8 @@ __device__ __forceinline__ double iceStrength(double Pstar, double hg, double ag
 }
 // The strength depends only on h and A, which the subcycles do not change: evaluated once
 // per timestep at the Gauss points of every element, Pg[q * ne + e], q = gx + G gy.
-template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_strength(int ne, const double* h, const double* A,
-    double Pstar, double* Pg)
+template <int CG, int DG, class T>
+__global__ __launch_bounds__(kBlock) void k_strength(int ne, const double* h, const double* A, double Pstar, T* Pg)
 {
     constexpr int NG = MomLayout<CG>::G;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -421,8 +424,16 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_strength(i
                 hg += hc[i] * b[i];
                 ag += ac[i] * b[i];
             }
-            Pg[static_cast<size_t>(gx + NG * gy) * ne + e] = iceStrength(Pstar, hg, ag);
+            Pg[static_cast<size_t>(gx + NG * gy) * ne + e] = static_cast<T>(iceStrength(Pstar, hg, ag));
         }
+}
+
+// Element-wise conversion between the FP64 stress and its FP32 copy.
+template <class S, class D> __global__ __launch_bounds__(kBlock) void k_cast(const S* src, D* dst, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = static_cast<D>(src[i]);
 }
 
 // stressUpdateHighOrder for one element (MEVPStressUpdateStep.hpp:30-118): sigma <-
@@ -505,49 +516,158 @@ __device__ __forceinline__ constexpr double invMassRef(int s)
 {
     return s == 0 ? 1.0 : (s <= 2 ? 12.0 : (s <= 4 ? 180.0 : (s == 5 ? 144.0 : 2160.0)));
 }
-template <int CG, int DG>
+// Element integrals are accumulated in R: plainly in FP64, with Kahan compensation in FP32
+// (the exact sum is s - c; initProduct keeps the rounding error of a product as well).
+template <class R> struct Acc {
+    R s;
+    __device__ __forceinline__ void init(R v) { s = v; }
+    __device__ __forceinline__ void initProduct(R a, R b) { s = a * b; }
+    __device__ __forceinline__ void add(R x) { s += x; }
+    __device__ __forceinline__ R get() const { return s; }
+    __device__ __forceinline__ double exact() const { return static_cast<double>(s); }
+};
+template <> struct Acc<float> {
+    float s, c;
+    __device__ __forceinline__ void init(float v)
+    {
+        s = v;
+        c = 0.0f;
+    }
+    __device__ __forceinline__ void initProduct(float a, float b)
+    {
+        s = a * b;
+        c = -fmaf(a, b, -s);
+    }
+    __device__ __forceinline__ void add(float x)
+    {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    // adds a * b with the rounding error of the product kept in the compensation
+    __device__ __forceinline__ void addProduct(float a, float b)
+    {
+        const float p = a * b;
+        const float err = fmaf(a, b, -p);
+        add(p);
+        c -= err;
+    }
+    __device__ __forceinline__ float get() const { return s - c; }
+    __device__ __forceinline__ double exact() const { return static_cast<double>(s) - static_cast<double>(c); }
+};
+
+// The mEVP stress relaxation sigma <- (1 - 1/alpha) sigma + ... amplifies a bias in the
+// rounding of the stored stress by up to alpha (near a fixed point round-to-nearest makes
+// the same error every subcycle). The FP32 stress is therefore stored with unbiased
+// stochastic rounding: x is rounded away from zero with probability (|x| - |rz(x)|) / ulp,
+// the random bits a hash of (element, coefficient, subcycle), so runs are reproducible.
+__device__ __forceinline__ uint32_t mixBits(uint32_t e, uint32_t k, uint32_t it)
+{
+    uint32_t h = e * 0x9E3779B1u ^ (k + 1u) * 0x85EBCA77u ^ (it + 1u) * 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ float roundStochastic(double x, uint32_t bits)
+{
+    const float f = __double2float_rz(x);
+    const double r = fabs(x - static_cast<double>(f));
+    const float g = nextafterf(f, x > 0.0 ? INFINITY : -INFINITY);
+    const double ulp = fabs(static_cast<double>(g) - static_cast<double>(f));
+    return static_cast<double>(bits >> 8) * 0x1p-24 * ulp < r ? g : f;
+}
+template <class R, class C> __device__ __forceinline__ R storeValue(const Acc<C>& a, int e, int k, uint32_t it)
+{
+    if constexpr (sizeof(R) == 8)
+        return a.get();
+    else
+        return roundStochastic(a.exact(), mixBits(static_cast<uint32_t>(e), static_cast<uint32_t>(k), it));
+}
+__device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+__device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
+template <class R> __device__ __forceinline__ R* stressOf(const ElemArgs& a);
+template <> __device__ __forceinline__ double* stressOf<double>(const ElemArgs& a) { return a.sig; }
+template <> __device__ __forceinline__ float* stressOf<float>(const ElemArgs& a) { return a.sigF; }
+template <class R> __device__ __forceinline__ const R* strengthOf(const ElemArgs& a);
+template <> __device__ __forceinline__ const double* strengthOf<double>(const ElemArgs& a) { return a.Pg; }
+template <> __device__ __forceinline__ const float* strengthOf<float>(const ElemArgs& a) { return a.PgF; }
+
+// R = double: the FP64 path. R = float: strain, Delta, stress and the projection in FP32;
+// the FP64 node velocities enter relative to the element's centre node (their differences,
+// which the gradient needs, are formed exactly in FP64 before rounding to FP32).
+template <int CG, int DG, class R, class C = R>
 __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
     const size_t ne = a.ne;
     const bool oc = a.ocean[e] != 0;
-    double buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
+    R* const sig = stressOf<R>(a);
+    const R* const Pg = strengthOf<R>(a);
+    C buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
     if (oc) {
         const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
         const int cgi = CG * row * iy + CG * ix;
+        double lu[N], lv[N];
 #pragma unroll
         for (int r = 0; r < K; ++r)
 #pragma unroll
             for (int c = 0; c < K; ++c) {
-                buf[c + K * r] = a.u[cgi + c + r * row];
-                buf[N + c + K * r] = a.v[cgi + c + r * row];
+                lu[c + K * r] = a.u[cgi + c + r * row];
+                lv[c + K * r] = a.v[cgi + c + r * row];
             }
+        const double cu = sizeof(C) == 8 ? 0.0 : lu[N / 2], cv = sizeof(C) == 8 ? 0.0 : lv[N / 2];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            buf[j] = static_cast<C>(lu[j] - cu);
+            buf[N + j] = static_cast<C>(lv[j] - cv);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 3 * S; ++k)
-            buf[k] = a.eps[k * ne + e];
+            buf[k] = static_cast<C>(a.eps[k * ne + e]);
     }
-    // previous stress, relaxed in place: the accumulators start at (1 - 1/alpha) sigma
-    double acc1[S], acc2[S], acc3[S];
+    const C xiX = static_cast<C>(a.xiX), etaX = static_cast<C>(a.etaX);
+    const C xiY = static_cast<C>(a.xiY), etaY = static_cast<C>(a.etaY);
+    const C DM2 = static_cast<C>(a.DM2);
+    // previous stress, relaxed in place. FP64: the accumulators start at (1 - 1/alpha) sigma,
+    // the reference's form. FP32: sigma - sigma/alpha with the same rounded 1/alpha that
+    // scales the new term, so that the fixed point is the projected sigma_VP exactly (a
+    // separately rounded 1 - 1/alpha would move it by up to 2^-24 alpha relative).
+    Acc<C> acc1[S], acc2[S], acc3[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        acc1[s] = a.sig[(0 * S + s) * ne + e] * a.relax;
-        acc2[s] = a.sig[(1 * S + s) * ne + e] * a.relax;
-        acc3[s] = a.sig[(2 * S + s) * ne + e] * a.relax;
+        const C o1 = sig[(0 * S + s) * ne + e], o2 = sig[(1 * S + s) * ne + e], o3 = sig[(2 * S + s) * ne + e];
+        if constexpr (sizeof(C) == 8) {
+            acc1[s].initProduct(o1, a.relax);
+            acc2[s].initProduct(o2, a.relax);
+            acc3[s].initProduct(o3, a.relax);
+        } else {
+            const C mInvAlpha = -static_cast<C>(a.invAlpha);
+            acc1[s].init(o1);
+            acc2[s].init(o2);
+            acc3[s].init(o3);
+            acc1[s].addProduct(o1, mInvAlpha);
+            acc2[s].addProduct(o2, mInvAlpha);
+            acc3[s].addProduct(o3, mInvAlpha);
+        }
     }
 #pragma unroll
     for (int gy = 0; gy < NG; ++gy) {
         // velocity interpolated (Iu, Iv) and differentiated (Du, Dv) along y at this Gauss row
-        double Iu[K], Du[K], Iv[K], Dv[K];
+        C Iu[K], Du[K], Iv[K], Dv[K];
         if (oc) {
 #pragma unroll
             for (int jx = 0; jx < K; ++jx) {
-                double iu = 0, du = 0, iv = 0, dv = 0;
+                C iu = 0, du = 0, iv = 0, dv = 0;
 #pragma unroll
                 for (int jy = 0; jy < K; ++jy) {
-                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy));
-                    const double dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+                    const C ph = static_cast<C>(cg_1d(CG, jy, gauss_point(NG, gy)));
+                    const C dp = static_cast<C>(cg_1d_dx(CG, jy, gauss_point(NG, gy)));
                     iu += ph * buf[jx + K * jy];
                     du += dp * buf[jx + K * jy];
                     iv += ph * buf[N + jx + K * jy];
@@ -561,55 +681,56 @@ __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
         }
 #pragma unroll
         for (int gx = 0; gx < NG; ++gx) {
-            double b[8];
-            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, b);
-            double g11 = 0, g12 = 0, g22 = 0;
+            double bd[8];
+            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
+            C g11 = 0, g12 = 0, g22 = 0;
             if (oc) {
-                double uxi = 0, ueta = 0, vxi = 0, veta = 0;
+                C uxi = 0, ueta = 0, vxi = 0, veta = 0;
 #pragma unroll
                 for (int jx = 0; jx < K; ++jx) {
-                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx));
-                    const double dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                    const C ph = static_cast<C>(cg_1d(CG, jx, gauss_point(NG, gx)));
+                    const C dp = static_cast<C>(cg_1d_dx(CG, jx, gauss_point(NG, gx)));
                     uxi += dp * Iu[jx];
                     ueta += ph * Du[jx];
                     vxi += dp * Iv[jx];
                     veta += ph * Dv[jx];
                 }
-                g11 = uxi * a.xiX + ueta * a.etaX;
-                g22 = vxi * a.xiY + veta * a.etaY;
-                g12 = 0.5 * ((uxi * a.xiY + ueta * a.etaY) + (vxi * a.xiX + veta * a.etaX));
+                g11 = uxi * xiX + ueta * etaX;
+                g22 = vxi * xiY + veta * etaY;
+                g12 = C(0.5) * ((uxi * xiY + ueta * etaY) + (vxi * xiX + veta * etaX));
             } else {
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    g11 += buf[s] * b[s];
-                    g12 += buf[S + s] * b[s];
-                    g22 += buf[2 * S + s] * b[s];
+                    const C b = static_cast<C>(bd[s]);
+                    g11 += buf[s] * b;
+                    g12 += buf[S + s] * b;
+                    g22 += buf[2 * S + s] * b;
                 }
             }
-            const double X = a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12;
-            const double P = a.Pg[(gx + NG * gy) * ne + e];
-            const double c = (P * 0.125) * rsqrt(X); // P / (8 Delta)
-            const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.invAlpha;
-            const double r11 = w * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
-            const double r12 = w * ((2.0 * c) * g12);
-            const double r22 = w * (c * (5.0 * g22 + 3.0 * g11) - 0.5 * P);
+            const C X = DM2 + C(1.25) * (g11 * g11 + g22 * g22) + C(1.50) * g11 * g22 + g12 * g12;
+            const C P = Pg[(gx + NG * gy) * ne + e];
+            const C c = (P * C(0.125)) * rsq(X); // P / (8 Delta)
+            const C w = static_cast<C>(gauss_weight(NG, gx) * gauss_weight(NG, gy)) * static_cast<C>(a.invAlpha);
+            const C r11 = w * (c * (C(5.0) * g11 + C(3.0) * g22) - C(0.5) * P);
+            const C r12 = w * ((C(2.0) * c) * g12);
+            const C r22 = w * (c * (C(5.0) * g22 + C(3.0) * g11) - C(0.5) * P);
             // iMJwPSI[s][q] = PSI_s(q) w_q / m_s: w_q is in r, PSI_s(q) / m_s is a literal
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const double bs = b[s] * invMassRef(s);
+                const double bs = bd[s] * invMassRef(s);
                 if (bs == 0.0) // basis functions vanishing at this Gauss point (a literal test)
                     continue;
-                acc1[s] += bs * r11;
-                acc2[s] += bs * r12;
-                acc3[s] += bs * r22;
+                acc1[s].add(static_cast<C>(bs) * r11);
+                acc2[s].add(static_cast<C>(bs) * r12);
+                acc3[s].add(static_cast<C>(bs) * r22);
             }
         }
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        a.sig[(0 * S + s) * ne + e] = acc1[s];
-        a.sig[(1 * S + s) * ne + e] = acc2[s];
-        a.sig[(2 * S + s) * ne + e] = acc3[s];
+        sig[(0 * S + s) * ne + e] = storeValue<R>(acc1[s], e, 0 * S + s, a.iter);
+        sig[(1 * S + s) * ne + e] = storeValue<R>(acc2[s], e, 1 * S + s, a.iter);
+        sig[(2 * S + s) * ne + e] = storeValue<R>(acc3[s], e, 2 * S + s, a.iter);
     }
 }
 
@@ -621,7 +742,7 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
     if (e >= a.ne)
         return;
     if constexpr (UNI) {
-        elementAffine<CG, DG>(a, e);
+        elementAffine<CG, DG, double>(a, e);
     } else {
         double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
         if (a.ocean[e])
@@ -631,6 +752,14 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
         stressLocal<CG, DG, UNI, true>(a, e, e11, e12, e22, s11, s12, s22);
         storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
     }
+}
+// FP32 variant of the fused pass (uniform meshes).
+template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_element_f32(ElemArgs a)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne)
+        return;
+    elementAffine<CG, DG, float, float>(a, e);
 }
 template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_strain(ElemArgs a)
 {
@@ -660,6 +789,7 @@ struct NodeArgs {
     const uint8_t* ocean;
     const uint8_t* dirNode;
     const double* sig;
+    const float* sigF; // FP32 variant
     const double* ops;
     size_t stride;
     double* u;
@@ -751,56 +881,74 @@ __device__ __forceinline__ void divergenceLocal(const NodeArgs& a, int e, double
 // factorisation: stress at the Gauss points, the Jacobian-weighted fluxes
 // T = w |J| J^-T-rows applied to (s11, s12) and (s12, s22), then
 // tx_j = sum_q (PHI_xi[j](q) Tx_xi(q) + PHI_eta[j](q) Tx_eta(q)), factorised per Gauss row.
-template <int CG>
-__device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, double* tx, double* ty)
+// Src: the stored stress type (a.sig for double, a.sigF for float); R: the arithmetic.
+template <int CG, class R, class Src>
+__device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, R* tx, R* ty)
 {
     constexpr int S = cg2dgstress(CG), NG = gp1d(S), N = cgdofs(CG), K = CG + 1;
-    double s11[S], s12[S], s22[S];
-    loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    R s11[S], s12[S], s22[S];
+    if constexpr (sizeof(Src) == 8)
+        loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    else
+        loadTensor<S>(a.sigF, a.ne, e, s11, s12, s22);
+    const R xiX = static_cast<R>(a.xiX), etaX = static_cast<R>(a.etaX);
+    const R xiY = static_cast<R>(a.xiY), etaY = static_cast<R>(a.etaY);
+    Acc<R> sx[N], sy[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-        tx[j] = ty[j] = 0.0;
+    for (int j = 0; j < N; ++j) {
+        sx[j].init(0);
+        sy[j].init(0);
+    }
 #pragma unroll
     for (int gy = 0; gy < NG; ++gy) {
-        double ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
+        Acc<R> ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
 #pragma unroll
-        for (int jx = 0; jx < K; ++jx)
-            ax[jx] = bx[jx] = ay[jx] = by[jx] = 0.0;
+        for (int jx = 0; jx < K; ++jx) {
+            ax[jx].init(0);
+            bx[jx].init(0);
+            ay[jx].init(0);
+            by[jx].init(0);
+        }
 #pragma unroll
         for (int gx = 0; gx < NG; ++gx) {
             double b[8];
             dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, b);
-            double v11 = 0, v12 = 0, v22 = 0;
+            R v11 = 0, v12 = 0, v22 = 0;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                v11 += s11[s] * b[s];
-                v12 += s12[s] * b[s];
-                v22 += s22[s] * b[s];
+                v11 += s11[s] * static_cast<R>(b[s]);
+                v12 += s12[s] * static_cast<R>(b[s]);
+                v22 += s22[s] * static_cast<R>(b[s]);
             }
-            const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.detJ;
+            const R w = static_cast<R>(gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.detJ);
             // grad phi = (phi_xi xiX + phi_eta etaX, phi_xi xiY + phi_eta etaY)
-            const double fxXi = w * (a.xiX * v11 + a.xiY * v12), fxEta = w * (a.etaX * v11 + a.etaY * v12);
-            const double fyXi = w * (a.xiX * v12 + a.xiY * v22), fyEta = w * (a.etaX * v12 + a.etaY * v22);
+            const R fxXi = w * (xiX * v11 + xiY * v12), fxEta = w * (etaX * v11 + etaY * v12);
+            const R fyXi = w * (xiX * v12 + xiY * v22), fyEta = w * (etaX * v12 + etaY * v22);
 #pragma unroll
             for (int jx = 0; jx < K; ++jx) {
-                const double ph = cg_1d(CG, jx, gauss_point(NG, gx));
-                const double dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
-                ax[jx] += dp * fxXi;  // d/dxi part, integrated along x
-                bx[jx] += ph * fxEta; // d/deta part
-                ay[jx] += dp * fyXi;
-                by[jx] += ph * fyEta;
+                const R ph = static_cast<R>(cg_1d(CG, jx, gauss_point(NG, gx)));
+                const R dp = static_cast<R>(cg_1d_dx(CG, jx, gauss_point(NG, gx)));
+                ax[jx].add(dp * fxXi);  // d/dxi part, integrated along x
+                bx[jx].add(ph * fxEta); // d/deta part
+                ay[jx].add(dp * fyXi);
+                by[jx].add(ph * fyEta);
             }
         }
 #pragma unroll
         for (int jy = 0; jy < K; ++jy) {
-            const double ph = cg_1d(CG, jy, gauss_point(NG, gy));
-            const double dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+            const R ph = static_cast<R>(cg_1d(CG, jy, gauss_point(NG, gy)));
+            const R dp = static_cast<R>(cg_1d_dx(CG, jy, gauss_point(NG, gy)));
 #pragma unroll
             for (int jx = 0; jx < K; ++jx) {
-                tx[jx + K * jy] += ph * ax[jx] + dp * bx[jx];
-                ty[jx + K * jy] += ph * ay[jx] + dp * by[jx];
+                sx[jx + K * jy].add(ph * ax[jx].get() + dp * bx[jx].get());
+                sy[jx + K * jy].add(ph * ay[jx].get() + dp * by[jx].get());
             }
         }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        tx[j] = sx[j].get();
+        ty[j] = sy[j].get();
     }
 }
 
@@ -860,11 +1008,12 @@ template <int CG> struct NodeTile {
 };
 
 // Fused stressDivergence + updateMomentum + applyBoundaries (no periodic segments).
-template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
+template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
 {
+    static_assert(UNI || sizeof(R) == 8, "the FP32 node pass needs a uniform mesh");
     using T = NodeTile<CG>;
     constexpr int N = T::N;
-    __shared__ double contrib[T::HX * T::HY][N][2];
+    __shared__ R contrib[T::HX * T::HY][N][2];
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
     const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
     if (tile < 0)
@@ -874,15 +1023,15 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_nod
     // Phase 1: weak-divergence contributions of the tile and its left/bottom halo.
     for (int le = threadIdx.x; le < T::HX * T::HY; le += kBlock) {
         const int ex = ex0 - 1 + le % T::HX, ey = ey0 - 1 + le / T::HX;
-        double tx[N], ty[N];
+        R tx[N], ty[N];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            tx[j] = ty[j] = 0.0;
+            tx[j] = ty[j] = 0;
         if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
             const int e = ex + a.nx * ey;
             if (a.ocean[e]) {
                 if constexpr (UNI)
-                    divergenceAffine<CG>(a, e, tx, ty);
+                    divergenceAffine<CG, R, R>(a, e, tx, ty);
                 else
                     divergenceLocal<CG, false>(a, e, tx, ty);
             }
@@ -927,8 +1076,8 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_mevp_nod
                     double dx = 0.0, dy = 0.0;
                     forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
                         const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
-                        dx -= contrib[le][j][0];
-                        dy -= contrib[le][j][1];
+                        dx -= static_cast<double>(contrib[le][j][0]);
+                        dy -= static_cast<double>(contrib[le][j][1]);
                     });
                     momentumFrozen(a, in[b], dx, dy, un, vn);
                 }
@@ -1111,6 +1260,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<uint8_t> dirNode;
     DBuf<double2> fzOcn, fzCf, fzQ; // frozen momentum coefficients (k_node_coeffs)
     DBuf<double> fzIML, strength;   // 1 / ML; ice strength at the Gauss points
+    DBuf<float> sigF, strengthF;    // FP32 variant: stress and strength
+    uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
 
     explicit DynImpl(const nsdg_params& prm)
@@ -1193,8 +1344,19 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         dirNode.alloc(nn);
         dirNode.upload(dn.data(), nn, s);
         passive.clear();
+        subcycleCount = 0;
         NSDG_HIP(hipStreamSynchronize(s));
         ready = true;
+        if (precision == NSDG_PREC_FP32) {
+            try {
+                checkPrecision();
+            } catch (...) {
+                precision = NSDG_PREC_FP64;
+                throw;
+            }
+            sigF.release();
+            allocPrecision();
+        }
     }
 
     // ---- field access
@@ -1395,6 +1557,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.ops = mops.p;
         a.stride = opStride();
         a.Pg = strength.p;
+        a.sigF = sigF.p;
+        a.PgF = strengthF.p;
+        a.iter = subcycleCount;
         a.Pstar = p.Pstar;
         a.DM2 = p.DeltaMin * p.DeltaMin;
         a.invAlpha = 1.0 / p.alpha;
@@ -1415,6 +1580,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.ocean = mesh.ocean.p;
         a.dirNode = dirNode.p;
         a.sig = sig.p;
+        a.sigF = sigF.p;
         a.ops = mops.p;
         a.stride = opStride();
         a.u = u.p;
@@ -1454,7 +1620,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     {
         Timed t(prof, s, "mevp_element");
         const ElemArgs a = elemArgs();
-        if (mesh.uniform)
+        if (precision == NSDG_PREC_FP32) {
+            require(sigF.n == 3 * S * ne && strengthF.n == static_cast<size_t>(L::Q) * ne,
+                "FP32 buffers not allocated", NSDG_ERR_STATE);
+            k_mevp_element_f32<CG, DG><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        }
+        else if (mesh.uniform)
             k_mevp_element<CG, DG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
         else
             k_mevp_element<CG, DG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
@@ -1467,10 +1638,15 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             Timed t(prof, s, "mevp_node");
             using T = NodeTile<CG>;
             const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-            if (mesh.uniform)
-                k_mevp_node<CG, true><<<grid, kBlock, 0, s>>>(a);
+            if (precision == NSDG_PREC_FP32) {
+                require(mesh.uniform && sigF.n == 3 * S * ne, "FP32 node pass needs its buffers and a uniform mesh",
+                    NSDG_ERR_STATE);
+                k_mevp_node<CG, true, float><<<grid, kBlock, 0, s>>>(a);
+            }
+            else if (mesh.uniform)
+                k_mevp_node<CG, true, double><<<grid, kBlock, 0, s>>>(a);
             else
-                k_mevp_node<CG, false><<<grid, kBlock, 0, s>>>(a);
+                k_mevp_node<CG, false, double><<<grid, kBlock, 0, s>>>(a);
             checkLaunch("mevp_node");
             return;
         }
@@ -1515,18 +1691,32 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     void frozenAsync()
     {
         Timed t(prof, s, "frozen_terms");
-        k_strength<CG, DG><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar, strength.p);
+        if (precision == NSDG_PREC_FP32)
+            k_strength<CG, DG, float><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar, strengthF.p);
+        else
+            k_strength<CG, DG, double><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar, strength.p);
         checkLaunch("strength");
         k_node_coeffs<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), fzOcn.p, fzCf.p, fzQ.p, fzIML.p);
         checkLaunch("node_coeffs");
     }
     void subcyclesAsync(int n)
     {
-        if (n > 0)
-            frozenAsync();
+        if (n <= 0)
+            return;
+        frozenAsync();
+        const bool f32 = precision == NSDG_PREC_FP32;
+        if (f32) {
+            k_cast<double, float><<<blocksFor(sig.n), kBlock, 0, s>>>(sig.p, sigF.p, sig.n);
+            checkLaunch("stress_to_fp32");
+        }
         for (int k = 0; k < n; ++k) {
             elementPass();
             nodePass();
+            ++subcycleCount;
+        }
+        if (f32) {
+            k_cast<float, double><<<blocksFor(sig.n), kBlock, 0, s>>>(sigF.p, sig.p, sig.n);
+            checkLaunch("stress_to_fp64");
         }
     }
 
@@ -1607,11 +1797,35 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
         subcyclesAsync(n);
     }
+    // FP32 stress storage and element arithmetic exist for the sum-factorised kernels of
+    // uniform meshes with the fused node pass (no periodic segments).
+    void checkPrecision() const
+    {
+        if (precision == NSDG_PREC_FP32)
+            require(mesh.uniform && !mesh.hasPeriodic,
+                "the FP32 variant needs a uniform (affine) mesh without periodic segments", NSDG_ERR_UNSUPPORTED);
+    }
+    void allocPrecision()
+    {
+        if (precision == NSDG_PREC_FP32 && ready && sigF.n != 3 * S * ne) {
+            sigF.alloc(3 * S * ne);
+            strengthF.alloc(static_cast<size_t>(L::Q) * ne);
+        }
+    }
     void setPrecision(int pr) override
     {
         require(pr == NSDG_PREC_FP64 || pr == NSDG_PREC_FP32, "precision must be NSDG_PREC_FP64 or NSDG_PREC_FP32");
-        require(pr == NSDG_PREC_FP64, "FP32 element pass not built", NSDG_ERR_UNSUPPORTED);
+        const int old = precision;
         precision = pr;
+        if (ready) {
+            try {
+                checkPrecision();
+            } catch (...) {
+                precision = old;
+                throw;
+            }
+            allocPrecision();
+        }
     }
     void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) override
     {

@@ -142,6 +142,9 @@ void DeviceMesh::build(const HostMesh& m, hipStream_t s)
         etaX = -yXi / detJ;
         etaY = xXi / detJ;
     }
+    // congruent translates on a structured grid are parallelograms; a degenerate or
+    // inverted element 0 leaves the per-element operator path
+    uniform = affine;
 
     std::vector<uint32_t> fl(ne, 0);
     std::vector<double> fX(m.nx * (m.ny + 1), 1.0), fY((m.nx + 1) * m.ny, 1.0);
