@@ -167,6 +167,16 @@ int nsdg_dyn_reset_dg0(nsdg_dyn* d, const char* name);
 /* FP64 conservation sums (Tools::MeanValue, Tools.hpp:24-45) of hice and cice over the
  * ocean elements, with compensated (two-sum) accumulation on the device. */
 int nsdg_dyn_mass(nsdg_dyn* d, double* ice_volume, double* ice_area);
+/* Momentum residual of the current velocity (no counterpart in the reference, which
+ * runs a fixed subcycle count): R = M_L [c1 (u - u0) - A (tau_a + F_oc |u_o - u| (u_o - u))
+ * + rho H f_c (u | -v) + rho H g grad(eta)] - dS(Pi sigma_VP(eps(u))) per CG node, the
+ * fixed-point equation of updateMomentum (VPCGDynamicsKernel.hpp:132-172) with the stress
+ * of MEVPStressUpdateStep.hpp:30-118 at alpha = 1. *r = |R|_2 / |M_L A tau_a|_2 over the
+ * non-Dirichlet nodes; the two norms go to *norm_r, *norm_forcing (any may be NULL). */
+int nsdg_dyn_residual(nsdg_dyn* d, double* r, double* norm_r, double* norm_forcing);
+/* stress <- Pi sigma_VP(eps(u)): the converged mEVP stress of the current velocity
+ * (a warm start for the stress that matches a given velocity). */
+int nsdg_dyn_stress_from_velocity(nsdg_dyn* d);
 int nsdg_dyn_synchronize(nsdg_dyn* d);
 /* Device state (all DG fields including stress, CG velocities) as an opaque blob in this
  * library's own format. With buf == NULL, *size receives the required size. */

@@ -374,6 +374,14 @@ int nsdg_dyn_mass(nsdg_dyn* d, double* vol, double* area)
 {
     return guard([&] { nonNull(d, "dyn")->k->mass(vol, area); });
 }
+int nsdg_dyn_residual(nsdg_dyn* d, double* r, double* norm_r, double* norm_forcing)
+{
+    return guard([&] { nonNull(d, "dyn")->k->residual(r, norm_r, norm_forcing); });
+}
+int nsdg_dyn_stress_from_velocity(nsdg_dyn* d)
+{
+    return guard([&] { nonNull(d, "dyn")->k->stressFromVelocity(); });
+}
 int nsdg_dyn_synchronize(nsdg_dyn* d)
 {
     return guard([&] { nonNull(d, "dyn")->k->synchronize(); });

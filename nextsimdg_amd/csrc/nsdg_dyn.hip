@@ -438,7 +438,7 @@ template <class S, class D> __global__ __launch_bounds__(kBlock) void k_cast(con
 
 // stressUpdateHighOrder for one element (MEVPStressUpdateStep.hpp:30-118): sigma <-
 // (1 - 1/alpha) sigma + iMJwPSI (1/alpha) sigma_VP(strain, h, A) at the Gauss points.
-template <int CG, int DG, bool UNI, bool PG>
+template <int CG, int DG, bool UNI, bool PG, bool VP = false>
 __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const double* e11, const double* e12,
     const double* e22, double* s11, double* s12, double* s22)
 {
@@ -491,6 +491,15 @@ __device__ __forceinline__ void stressLocal(const ElemArgs& a, int e, const doub
             acc2[s] += b * r12;
             acc3[s] += b * r22;
         }
+    }
+    if constexpr (VP) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            s11[s] = acc1[s];
+            s12[s] = acc2[s];
+            s22[s] = acc3[s];
+        }
+        return;
     }
     // the previous stress is read only now, keeping its 3 S registers free above
     loadTensor<S>(a.sig, ne, e, s11, s12, s22);
@@ -599,7 +608,9 @@ template <> __device__ __forceinline__ const float* strengthOf<float>(const Elem
 // R = double: the FP64 path. R = float: strain, Delta, stress and the projection in FP32;
 // the FP64 node velocities enter relative to the element's centre node (their differences,
 // which the gradient needs, are formed exactly in FP64 before rounding to FP32).
-template <int CG, int DG, class R, class C = R>
+// VP: sigma <- Pi sigma_VP(eps(u)) outright (no relaxation, the old stress is not read);
+// used for the momentum residual and for the stress of a predicted velocity.
+template <int CG, int DG, class R, class C = R, bool VP = false>
 __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
 {
     using L = MomLayout<CG>;
@@ -641,6 +652,12 @@ __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
     Acc<C> acc1[S], acc2[S], acc3[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+        if constexpr (VP) {
+            acc1[s].init(0);
+            acc2[s].init(0);
+            acc3[s].init(0);
+            continue;
+        }
         const C o1 = sig[(0 * S + s) * ne + e], o2 = sig[(1 * S + s) * ne + e], o3 = sig[(2 * S + s) * ne + e];
         if constexpr (sizeof(C) == 8) {
             acc1[s].initProduct(o1, a.relax);
@@ -760,6 +777,25 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_eleme
     if (e >= a.ne)
         return;
     elementAffine<CG, DG, float, float>(a, e);
+}
+// sigma <- Pi sigma_VP(eps(u)) with a.invAlpha = 1 (the converged mEVP stress of u).
+template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_vp_stress(ElemArgs a)
+{
+    constexpr int S = cg2dgstress(CG);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne)
+        return;
+    if constexpr (UNI) {
+        elementAffine<CG, DG, double, double, true>(a, e);
+    } else {
+        double e11[S], e12[S], e22[S], s11[S], s12[S], s22[S];
+        if (a.ocean[e])
+            strainLocal<CG, UNI>(a, e, e11, e12, e22);
+        else
+            loadTensor<S>(a.eps, a.ne, e, e11, e12, e22);
+        stressLocal<CG, DG, UNI, true, true>(a, e, e11, e12, e22, s11, s12, s22);
+        storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    }
 }
 template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_strain(ElemArgs a)
 {
@@ -1140,6 +1176,81 @@ __global__ void k_boundaries(NodeArgs a)
     a.v[n] = 0.0;
 }
 
+// ---------------------------------------------------------------- momentum residual
+// Fixed-point residual of updateMomentum with the stress replaced by the converged VP
+// stress of the current velocity (dS = the divergence of Pi sigma_VP(eps(u))):
+//   R = M_L [c1 (u - u0) - A (tau_a + F_oc |du| (u_o - u)) + rho H f_c u + rho H g dEta/dx] - dS_x
+//   R_y likewise with - rho H f_c v (the same-component Coriolis term of the update),
+// c1 = rho H / dt, tau_a = F_atm |u_a| u_a. R = 0 exactly where the update
+// (VPCGDynamicsKernel.hpp:132-172) leaves u unchanged. Sums of R^2 and of (M_L A tau_a)^2
+// over the non-Dirichlet nodes: per-thread FP64 sums, wavefront shuffles, then one
+// partial per block, added in block order by k_residual_final (deterministic).
+constexpr int kResidualBlocks = 512;
+__device__ __forceinline__ double waveSum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        x += __shfl_xor(x, o, 64);
+    return x;
+}
+__global__ __launch_bounds__(kBlock) void k_residual_nodes(NodeArgs a, const double* dSx, const double* dSy,
+    double* part)
+{
+    __shared__ double red[2][kBlock / 64];
+    double r2 = 0.0, d2 = 0.0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.nn; n += gridDim.x * blockDim.x) {
+        if (a.dirNode[n])
+            continue;
+        const double u = a.u[n], v = a.v[n], uo = a.uo[n], vo = a.vo[n], ua = a.ua[n], va = a.va[n];
+        const double H = a.H[n], A = a.A[n], ML = a.ML[n];
+        const double absatm = sqrt(ua * ua + va * va);
+        const double du = uo - u, dv = v - vo;
+        const double absocn = sqrt(du * du + dv * dv);
+        const double c1 = a.rho_ice * H / a.dt, rhoH = a.rho_ice * H;
+        const double tax = a.F_atm * absatm * ua, tay = a.F_atm * absatm * va;
+        const double rx = ML
+                * (c1 * (u - a.u0[n]) - A * (tax + a.F_ocean * absocn * (uo - u)) + rhoH * a.fc * u
+                    + rhoH * a.gravity * a.gx[n])
+            - dSx[n];
+        const double ry = ML
+                * (c1 * (v - a.v0[n]) - A * (tay + a.F_ocean * absocn * (vo - v)) - rhoH * a.fc * v
+                    + rhoH * a.gravity * a.gy[n])
+            - dSy[n];
+        const double fx = ML * A * tax, fy = ML * A * tay;
+        r2 += rx * rx + ry * ry;
+        d2 += fx * fx + fy * fy;
+    }
+    r2 = waveSum(r2);
+    d2 = waveSum(d2);
+    const int w = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0) {
+        red[0][w] = r2;
+        red[1][w] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sr = 0.0, sd = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            sr += red[0][k];
+            sd += red[1][k];
+        }
+        part[2 * blockIdx.x] = sr;
+        part[2 * blockIdx.x + 1] = sd;
+    }
+}
+__global__ void k_residual_final(const double* part, int nb, double* out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    double sr = 0.0, sd = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        sr += part[2 * b];
+        sd += part[2 * b + 1];
+    }
+    out[0] = sr;
+    out[1] = sd;
+}
+
 // ---------------------------------------------------------------- outputs and inputs
 // getIceOceanStress (VPCGDynamicsKernel.hpp:96-120)
 __global__ void k_ice_ocean_stress(int nn, double F_ocean, const double* u, const double* v, const double* uo,
@@ -1262,6 +1373,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<double> fzIML, strength;   // 1 / ML; ice strength at the Gauss points
     DBuf<float> sigF, strengthF;    // FP32 variant: stress and strength
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
+    DBuf<double> sigVP, rpart, rsum; // momentum residual: VP stress, block partials, sums
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
 
     explicit DynImpl(const nsdg_params& prm)
@@ -1318,6 +1430,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         strength.alloc(static_cast<size_t>(L::Q) * ne);
         part.alloc(4 * blocksFor(ne));
         res.alloc(2);
+        sigVP.alloc(3 * S * ne);
+        deviceFill(sigVP.p, sigVP.n, 0.0, s);
+        rpart.alloc(2 * kResidualBlocks);
+        rsum.alloc(2);
         for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
             deviceFill(b->p, b->n, 0.0, s);
         for (auto* b : { &ssh, &eps, &sig, &u, &v, &u0, &v0, &uo, &vo, &ua, &va, &cgH, &cgA, &gX, &gY, &dSx, &dSy })
@@ -1796,6 +1912,68 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         need();
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
         subcyclesAsync(n);
+    }
+    // sigma_out <- Pi sigma_VP(eps(u)) (FP64, from the current h, A, u, v).
+    void vpStressAsync(double* out)
+    {
+        k_strength<CG, DG, double><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar, strength.p);
+        checkLaunch("strength");
+        ElemArgs a = elemArgs();
+        a.sig = out;
+        a.invAlpha = 1.0;
+        a.relax = 0.0;
+        if (mesh.uniform)
+            k_vp_stress<CG, DG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        else
+            k_vp_stress<CG, DG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        checkLaunch("vp_stress");
+    }
+    // r = |R| / |M_L A tau_a| over the non-Dirichlet nodes (see k_residual_nodes).
+    void residual(double* r, double* normR, double* normD) override
+    {
+        need();
+        require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
+        {
+            Timed t(prof, s, "residual");
+            vpStressAsync(sigVP.p);
+            NodeArgs a = nodeArgs();
+            a.sig = sigVP.p;
+            a.dSx = tmpCG.p;
+            a.dSy = tmpCG2.p;
+            if (mesh.uniform)
+                k_divergence<CG, true><<<blocksFor(nn), kBlock, 0, s>>>(a);
+            else
+                k_divergence<CG, false><<<blocksFor(nn), kBlock, 0, s>>>(a);
+            checkLaunch("residual_divergence");
+            for (const auto& seg : mesh.segments) {
+                k_periodic_average<CG><<<1, 1, 0, s>>>(
+                    mesh.nx, mesh.periodic.p + 4 * seg.first, static_cast<int>(seg.second), tmpCG.p, tmpCG2.p);
+                checkLaunch("periodic_average");
+            }
+            k_residual_nodes<<<kResidualBlocks, kBlock, 0, s>>>(a, tmpCG.p, tmpCG2.p, rpart.p);
+            checkLaunch("residual_nodes");
+            k_residual_final<<<1, 1, 0, s>>>(rpart.p, kResidualBlocks, rsum.p);
+            checkLaunch("residual_final");
+        }
+        double h2[2];
+        rsum.download(h2, 2, s);
+        NSDG_HIP(hipStreamSynchronize(s));
+        if (prof.on)
+            prof.collect();
+        const double nr = std::sqrt(h2[0]), nd = std::sqrt(h2[1]);
+        if (normR)
+            *normR = nr;
+        if (normD)
+            *normD = nd;
+        if (r)
+            *r = nd > 0 ? nr / nd : nr;
+    }
+    // sigma <- Pi sigma_VP(eps(u)): the converged mEVP stress of the current velocity.
+    void stressFromVelocity() override
+    {
+        need();
+        vpStressAsync(sig.p);
+        synchronize();
     }
     // FP32 stress storage and element arithmetic exist for the sum-factorised kernels of
     // uniform meshes with the fused node pass (no periodic segments).

@@ -29,6 +29,8 @@ struct DynBase {
     virtual void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) = 0;
     virtual void resetDG0(const std::string& name) = 0;
     virtual void mass(double* vol, double* area) = 0;
+    virtual void residual(double* r, double* normR, double* normD) = 0;
+    virtual void stressFromVelocity() = 0;
     virtual void synchronize() = 0;
     virtual std::vector<char> saveState() = 0;
     virtual void loadState(const char* buf, size_t size) = 0;

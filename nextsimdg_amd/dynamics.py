@@ -252,6 +252,17 @@ class MEVPDynamicsKernel:
         check(self.lib.nsdg_dyn_mass(self.h, ctypes.byref(v), ctypes.byref(a)))
         return v.value, a.value
 
+    def momentum_residual(self):
+        """(r, |R|, |M_L A tau_a|): the fixed-point residual of the momentum update with the
+        converged VP stress of the current velocity, over the non-Dirichlet nodes."""
+        r, nr, nd = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self.lib.nsdg_dyn_residual(self.h, ctypes.byref(r), ctypes.byref(nr), ctypes.byref(nd)))
+        return r.value, nr.value, nd.value
+
+    def stress_from_velocity(self):
+        """stress <- Pi sigma_VP(eps(u)), the converged mEVP stress of the current velocity."""
+        check(self.lib.nsdg_dyn_stress_from_velocity(self.h))
+
     def synchronize(self):
         check(self.lib.nsdg_dyn_synchronize(self.h))
 
