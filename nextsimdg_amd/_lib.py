@@ -1,4 +1,5 @@
-"""ctypes binding of libnsdg_hip.so (the C ABI declared in include/nsdg_dyn.h).
+"""ctypes binding of libnsdg_hip.so (the C ABI declared in include/nsdg_dyn.h and
+include/nsdg_cnn.h).
 
 There is no fallback: if the gfx950 library is missing or cannot be loaded, every
 entry point of this package raises.
@@ -79,6 +80,22 @@ SIGNATURES = {
     "nsdg_dyn_set_profiling": (_i, [_vp, _i]),
     "nsdg_dyn_kernel_stats": (_i, [_vp, _s, ctypes.POINTER(_i), _dp]),
     "nsdg_dyn_destroy": (_i, [_vp]),
+    # include/nsdg_cnn.h
+    "nsdg_cnn_create": (_vp, [ctypes.c_uint64]),
+    "nsdg_cnn_destroy": (_i, [_vp]),
+    "nsdg_cnn_param_count": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "nsdg_cnn_get_params": (_i, [_vp, _vp]),
+    "nsdg_cnn_set_params": (_i, [_vp, _vp]),
+    "nsdg_cnn_forward": (_i, [_vp, _vp, _i, _i, _i, _vp]),
+    "nsdg_cnn_loss_grad": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _dp, _vp]),
+    "nsdg_cnn_adam": (_i, [_vp, ctypes.c_float]),
+    "nsdg_cnn_set_profiling": (_i, [_vp, _i]),
+    "nsdg_cnn_kernel_stats": (_i, [_vp, _s, ctypes.POINTER(_i), _dp]),
+    "nsdg_dyn_features": (_i, [_vp, _vp]),
+    "nsdg_dyn_velocity_target": (_i, [_vp, _vp, _vp]),
+    "nsdg_dyn_apply_velocity": (_i, [_vp, _vp]),
+    "nsdg_dyn_warm_start": (_i, [_vp, _vp, _dp, _dp, ctypes.POINTER(_i)]),
+    "nsdg_dyn_subcycles_to_residual": (_i, [_vp, _d, _i, _i, ctypes.POINTER(_i), _dp]),
 }
 
 

@@ -1,8 +1,11 @@
 // extern "C" entry points of include/nsdg_dyn.h. Every call is guarded: C++
 // exceptions become a negative status and a per-thread message.
+#include "../../include/nsdg_cnn.h"
+#include "nsdg_cnn.hpp"
 #include "nsdg_dyn.hpp"
 #include "nsdg_transport.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 using namespace nsdg;
@@ -16,6 +19,10 @@ struct nsdg_transport {
 struct nsdg_dyn {
     std::unique_ptr<DynBase> k;
     int cg, dg;
+};
+struct nsdg_cnn {
+    std::unique_ptr<cnn::Net> net;
+    DBuf<uint16_t> x; // features of one timestep (warm start)
 };
 
 namespace {
@@ -429,6 +436,167 @@ int nsdg_dyn_destroy(nsdg_dyn* d)
 {
     delete d;
     return NSDG_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- learned warm start
+extern "C" {
+
+nsdg_cnn* nsdg_cnn_create(uint64_t seed)
+{
+    nsdg_cnn* h = nullptr;
+    if (guard([&] {
+            needDevice();
+            auto p = std::make_unique<nsdg_cnn>();
+            p->net = std::make_unique<cnn::Net>(seed);
+            h = p.release();
+        }))
+        return nullptr;
+    return h;
+}
+int nsdg_cnn_destroy(nsdg_cnn* net)
+{
+    return guard([&] { delete net; });
+}
+int nsdg_cnn_param_count(nsdg_cnn* net, size_t* count)
+{
+    return guard([&] {
+        nonNull(net, "net");
+        *nonNull(count, "count") = cnn::NPARAM;
+    });
+}
+int nsdg_cnn_get_params(nsdg_cnn* net, float* host)
+{
+    return guard([&] {
+        auto& n = *nonNull(net, "net")->net;
+        n.params.download(nonNull(host, "host"), cnn::NPARAM, n.s);
+        n.sync();
+    });
+}
+int nsdg_cnn_set_params(nsdg_cnn* net, const float* host)
+{
+    return guard([&] {
+        auto& n = *nonNull(net, "net")->net;
+        n.params.upload(nonNull(host, "host"), cnn::NPARAM, n.s);
+        n.pack();
+        n.sync();
+    });
+}
+int nsdg_cnn_forward(nsdg_cnn* net, const uint16_t* x, int n, int ny, int nx, float* y)
+{
+    return guard([&] {
+        auto& nt = *nonNull(net, "net")->net;
+        nt.forward(x, n, ny, nx, nonNull(y, "y"), false);
+        nt.sync();
+    });
+}
+int nsdg_cnn_loss_grad(nsdg_cnn* net, const uint16_t* x, const float* t, const float* w, int n, int ny, int nx,
+    double* loss, float* grad_host)
+{
+    return guard([&] {
+        auto& nt = *nonNull(net, "net")->net;
+        const double l = nt.lossGrad(x, t, w, n, ny, nx);
+        if (loss)
+            *loss = l;
+        if (grad_host) {
+            nt.grad.download(grad_host, cnn::NPARAM, nt.s);
+            nt.sync();
+        }
+    });
+}
+int nsdg_cnn_adam(nsdg_cnn* net, float lr)
+{
+    return guard([&] {
+        require(lr > 0.0f, "learning rate must be positive");
+        nonNull(net, "net")->net->adam(lr);
+    });
+}
+int nsdg_cnn_set_profiling(nsdg_cnn* net, int on)
+{
+    return guard([&] {
+        auto& nt = *nonNull(net, "net")->net;
+        nt.sync();
+        nt.prof.on = on != 0;
+        if (nt.prof.on)
+            nt.prof.stats.clear();
+    });
+}
+int nsdg_cnn_kernel_stats(nsdg_cnn* net, const char* kernel, int* count, double* total_ms)
+{
+    return guard([&] {
+        auto& nt = *nonNull(net, "net")->net;
+        nt.sync();
+        auto it = nt.prof.stats.find(nonNull(kernel, "kernel"));
+        if (count)
+            *count = it == nt.prof.stats.end() ? 0 : it->second.first;
+        if (total_ms)
+            *total_ms = it == nt.prof.stats.end() ? 0.0 : it->second.second;
+    });
+}
+
+int nsdg_dyn_features(nsdg_dyn* d, uint16_t* x)
+{
+    return guard([&] { nonNull(d, "dyn")->k->features(nonNull(x, "x")); });
+}
+int nsdg_dyn_velocity_target(nsdg_dyn* d, float* t, float* w)
+{
+    return guard([&] { nonNull(d, "dyn")->k->velocityTarget(nonNull(t, "t"), w); });
+}
+int nsdg_dyn_apply_velocity(nsdg_dyn* d, const float* y)
+{
+    return guard([&] { nonNull(d, "dyn")->k->applyVelocity(nonNull(y, "y")); });
+}
+int nsdg_dyn_warm_start(nsdg_dyn* d, nsdg_cnn* net, double* r_predicted, double* r_cold, int* used)
+{
+    return guard([&] {
+        auto& k = *nonNull(d, "dyn")->k;
+        auto& h = *nonNull(net, "net");
+        const int nx = k.gridNx(), ny = k.gridNy();
+        require(nx % 16 == 0, "the warm start needs nx to be a multiple of 16", NSDG_ERR_UNSUPPORTED);
+        const size_t ne = static_cast<size_t>(nx) * ny;
+        if (h.x.n < ne * cnn::C)
+            h.x.alloc(ne * cnn::C);
+        double rc = 0.0, rp = 0.0;
+        k.residual(&rc, nullptr, nullptr);
+        k.saveStart();
+        k.features(h.x.p);
+        h.net->forward(h.x.p, 1, ny, nx, nullptr, false);
+        h.net->sync();
+        k.applyVelocity(h.net->out.p);
+        k.stressFromVelocity();
+        k.residual(&rp, nullptr, nullptr);
+        const bool keep = rp <= rc;
+        if (!keep)
+            k.restoreStart();
+        if (r_predicted)
+            *r_predicted = rp;
+        if (r_cold)
+            *r_cold = rc;
+        if (used)
+            *used = keep ? 1 : 0;
+    });
+}
+int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, int* done,
+    double* r_final)
+{
+    return guard([&] {
+        auto& k = *nonNull(d, "dyn")->k;
+        require(max_subcycles >= 0 && check_every >= 1, "need max_subcycles >= 0 and check_every >= 1");
+        double r = 0.0;
+        k.residual(&r, nullptr, nullptr);
+        int n = 0;
+        while (r > r_target && n < max_subcycles) {
+            const int step = std::min(check_every, max_subcycles - n);
+            k.subcycles(step);
+            n += step;
+            k.residual(&r, nullptr, nullptr);
+        }
+        if (done)
+            *done = n;
+        if (r_final)
+            *r_final = r;
+    });
 }
 
 } // extern "C"

@@ -1251,6 +1251,100 @@ __global__ void k_residual_final(const double* part, int nb, double* out)
     out[1] = sd;
 }
 
+// ---------------------------------------------------------------- learned warm start
+// Network input and target on the element grid of a CG2 kernel. Element e = ix + nx iy
+// owns the CG nodes (2 ix + a, 2 iy + b), a, b in {0, 1}, listed k = a + 2 b; its centre
+// node is (2 ix + 1, 2 iy + 1). Scales bring every channel to order one.
+constexpr double kScaleU = 0.1;      // m/s, ice velocity and its change
+constexpr double kScaleWind = 10.0;  // m/s
+constexpr double kScaleOcean = 0.01; // m/s
+constexpr double kScaleStress = 1e4; // N/m
+__device__ __forceinline__ uint16_t bf16Of(double d)
+{
+    uint32_t u = __float_as_uint(static_cast<float>(d));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_features(int nx, int ny, const double* h, const double* A,
+    const uint8_t* ocean, const double* ua, const double* va, const double* uo, const double* vo, const double* u0,
+    const double* v0, const double* sig, double Pstar, uint16_t* x)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ne = nx * ny;
+    if (e >= ne)
+        return;
+    const int ix = e % nx, iy = e / nx, row = 2 * nx + 1;
+    const int centre = (2 * ix + 1) + row * (2 * iy + 1);
+    double f[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+        f[c] = 0.0;
+    const double hm = h[e], am = A[e];
+    f[0] = hm;
+    f[1] = am;
+    f[2] = ocean[e] ? 1.0 : 0.0;
+    f[3] = ua[centre] / kScaleWind;
+    f[4] = va[centre] / kScaleWind;
+    f[5] = uo[centre] / kScaleOcean;
+    f[6] = vo[centre] / kScaleOcean;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int n = (2 * ix + k % 2) + row * (2 * iy + k / 2);
+        f[7 + 2 * k] = u0[n] / kScaleU;
+        f[8 + 2 * k] = v0[n] / kScaleU;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        f[15 + c] = sig[static_cast<size_t>(c * S) * ne + e] / kScaleStress;
+    f[18] = Pstar * fmax(hm, 0.0) * exp(-20.0 * (1.0 - fmin(fmax(am, 0.0), 1.0))) / kScaleStress;
+    uint4* dst = reinterpret_cast<uint4*>(x + static_cast<size_t>(e) * 32);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            w[k] = static_cast<uint32_t>(bf16Of(f[8 * q + 2 * k])) | (static_cast<uint32_t>(bf16Of(f[8 * q + 2 * k + 1])) << 16);
+        dst[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_velocity_target(int nx, int ny, const double* u, const double* v,
+    const double* u0, const double* v0, const uint8_t* ocean, float* t, float* w)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nx * ny)
+        return;
+    const int ix = e % nx, iy = e / nx, row = 2 * nx + 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int n = (2 * ix + k % 2) + row * (2 * iy + k / 2);
+        t[8 * static_cast<size_t>(e) + 2 * k] = static_cast<float>((u[n] - u0[n]) / kScaleU);
+        t[8 * static_cast<size_t>(e) + 2 * k + 1] = static_cast<float>((v[n] - v0[n]) / kScaleU);
+    }
+    if (w)
+        w[e] = ocean[e] ? 1.0f : 0.0f;
+}
+__global__ __launch_bounds__(kBlock) void k_apply_velocity(int nx, int ny, const float* y, const double* u0,
+    const double* v0, const uint8_t* dirNode, double* u, double* v)
+{
+    const int row = 2 * nx + 1;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= row * (2 * ny + 1))
+        return;
+    const int ix = n % row, iy = n / row;
+    double un = u0[n], vn = v0[n];
+    if (ix < 2 * nx && iy < 2 * ny) {
+        const size_t e = static_cast<size_t>(ix / 2) + static_cast<size_t>(nx) * (iy / 2);
+        const int k = ix % 2 + 2 * (iy % 2);
+        un += kScaleU * y[8 * e + 2 * k];
+        vn += kScaleU * y[8 * e + 2 * k + 1];
+    }
+    if (dirNode[n])
+        un = vn = 0.0;
+    u[n] = un;
+    v[n] = vn;
+}
+
 // ---------------------------------------------------------------- outputs and inputs
 // getIceOceanStress (VPCGDynamicsKernel.hpp:96-120)
 __global__ void k_ice_ocean_stress(int nn, double F_ocean, const double* u, const double* v, const double* uo,
@@ -1374,6 +1468,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<float> sigF, strengthF;    // FP32 variant: stress and strength
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
     DBuf<double> sigVP, rpart, rsum; // momentum residual: VP stress, block partials, sums
+    DBuf<double> uStart, vStart, sigStart; // saved start of a timestep (warm-start gate)
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
 
     explicit DynImpl(const nsdg_params& prm)
@@ -1967,6 +2062,60 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             *normD = nd;
         if (r)
             *r = nd > 0 ? nr / nd : nr;
+    }
+    int gridNx() const override { return mesh.nx; }
+    int gridNy() const override { return mesh.ny; }
+    void needCG2() const
+    {
+        need();
+        require(CG == 2, "the learned warm start is built for CG2 velocities", NSDG_ERR_UNSUPPORTED);
+    }
+    void features(uint16_t* x) override
+    {
+        needCG2();
+        require(x != nullptr && reinterpret_cast<uintptr_t>(x) % 16 == 0, "feature buffer must be 16-byte aligned");
+        k_features<S><<<blocksFor(ne), kBlock, 0, s>>>(mesh.nx, mesh.ny, h.p, A.p, mesh.ocean.p, ua.p, va.p, uo.p,
+            vo.p, u0.p, v0.p, sig.p, p.Pstar, x);
+        checkLaunch("features");
+        synchronize();
+    }
+    void velocityTarget(float* t, float* w) override
+    {
+        needCG2();
+        require(t != nullptr, "target buffer is NULL");
+        k_velocity_target<<<blocksFor(ne), kBlock, 0, s>>>(mesh.nx, mesh.ny, u.p, v.p, u0.p, v0.p, mesh.ocean.p, t, w);
+        checkLaunch("velocity_target");
+        synchronize();
+    }
+    void applyVelocity(const float* y) override
+    {
+        needCG2();
+        require(y != nullptr, "prediction buffer is NULL");
+        k_apply_velocity<<<blocksFor(nn), kBlock, 0, s>>>(mesh.nx, mesh.ny, y, u0.p, v0.p, dirNode.p, u.p, v.p);
+        checkLaunch("apply_velocity");
+        synchronize();
+    }
+    void saveStart() override
+    {
+        need();
+        if (uStart.n != nn) {
+            uStart.alloc(nn);
+            vStart.alloc(nn);
+            sigStart.alloc(sig.n);
+        }
+        deviceCopy(uStart.p, u.p, nn, s);
+        deviceCopy(vStart.p, v.p, nn, s);
+        deviceCopy(sigStart.p, sig.p, sig.n, s);
+        synchronize();
+    }
+    void restoreStart() override
+    {
+        need();
+        require(uStart.n == nn, "no saved start", NSDG_ERR_STATE);
+        deviceCopy(u.p, uStart.p, nn, s);
+        deviceCopy(v.p, vStart.p, nn, s);
+        deviceCopy(sig.p, sigStart.p, sig.n, s);
+        synchronize();
     }
     // sigma <- Pi sigma_VP(eps(u)): the converged mEVP stress of the current velocity.
     void stressFromVelocity() override

@@ -31,6 +31,14 @@ struct DynBase {
     virtual void mass(double* vol, double* area) = 0;
     virtual void residual(double* r, double* normR, double* normD) = 0;
     virtual void stressFromVelocity() = 0;
+    // learned warm start (CG2 element grid; see include/nsdg_cnn.h)
+    virtual int gridNx() const = 0;
+    virtual int gridNy() const = 0;
+    virtual void features(uint16_t* x) = 0;
+    virtual void velocityTarget(float* t, float* w) = 0;
+    virtual void applyVelocity(const float* y) = 0;
+    virtual void saveStart() = 0;
+    virtual void restoreStart() = 0;
     virtual void synchronize() = 0;
     virtual std::vector<char> saveState() = 0;
     virtual void loadState(const char* buf, size_t size) = 0;

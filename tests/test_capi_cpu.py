@@ -10,12 +10,12 @@ import pytest
 from nextsimdg_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "nsdg_dyn.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("nsdg_dyn.h", "nsdg_cnn.h")]
 
 
 def declared_symbols():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(nsdg_\w+)\s*\(", text, re.M)))
+    text = "".join(open(h).read() for h in HEADERS)
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*|nsdg_cnn\*)\s+(nsdg_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_abi():
