@@ -120,6 +120,54 @@ def fp32_variant(nx, steps, warmup, err_steps, seed=0):
     }
 
 
+WEIGHTS = os.path.join(ROOT, "nextsimdg_amd", "data", "warm_start_256.npy")
+
+
+def learned_vs_plain(nx, kinds=("heterogeneous", "leads", "plain"), seed=5000, spinup=2, check_every=25):
+    """Speed at equal momentum residual: plain FP64 mEVP (100 subcycles from the cold start)
+    against the FP32 path started by the trained network with the residual gate, iterating
+    until r <= r(plain). Both from the same saved state at the start of a timestep."""
+    import torch
+    from nextsimdg_amd import learned as L
+
+    if not os.path.exists(WEIGHTS):
+        return {"error": f"no trained weights at {os.path.relpath(WEIGHTS, ROOT)}"}
+    net = L.WarmStartNet(0)
+    net.set_params(__import__("numpy").load(WEIGHTS))
+    out = {"workload": f"{nx}x{nx} CG2/DG2 box, one timestep after {spinup} plain steps, seeded cyclone "
+                       f"(track {seed}+); network trained at 256x256 (include/nsdg_cnn.h)",
+           "check_every": check_every, "cases": []}
+    for i, kind in enumerate(kinds):
+        k, off = L.new_track(nx, seed + i, kind)
+        for st in range(spinup):
+            L.plain_step(k, off, st)
+        L.begin_step(k, off, spinup)
+        blob = k.save_state()
+        r_cold = k.momentum_residual()[0]
+        k.subcycles(100)  # warm
+        k.load_state(blob)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        k.subcycles(100)
+        k.synchronize()
+        t_plain = time.perf_counter() - t
+        r_plain = k.momentum_residual()[0]
+        k.load_state(blob)
+        k.set_precision(1)
+        t = time.perf_counter()
+        rp, rc, used = L.warm_start(k, net)
+        done, r = L.subcycles_to_residual(k, r_plain, 1000, check_every)
+        t_learned = time.perf_counter() - t
+        out["cases"].append({"ice": kind, "r_cold": float(f"{r_cold:.5e}"), "r_plain_100": float(f"{r_plain:.5e}"),
+                             "r_predicted": float(f"{rp:.5e}"), "prediction_kept": used,
+                             "fp32_subcycles_to_r_plain": done, "r_reached": float(f"{r:.5e}"),
+                             "t_plain_ms": round(t_plain * 1e3, 3), "t_learned_ms": round(t_learned * 1e3, 3),
+                             "speedup": round(t_plain / t_learned, 3)})
+        del k
+    out["speedup_min"] = min(c["speedup"] for c in out["cases"])
+    return out
+
+
 def cpu_baseline(nx, ny, seed=0, target_s=12.0):
     """The FP64 oracle (OpenMP C++ restatement of the reference) on the host cores:
     mEVP subcycles of the same box, timed until about target_s seconds."""
@@ -176,6 +224,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp32-nx", type=int, default=1024, help="FP32 variant size (0 skips it)")
     ap.add_argument("--fp32-error-steps", type=int, default=100)
+    ap.add_argument("--learned-nx", type=int, default=1024, help="learned vs plain size (0 skips it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -205,8 +254,13 @@ def main():
             traffic = json.load(open(args.traffic)).get(f"{dom}_{nx}")
         except (ValueError, OSError):
             traffic = None
+    # % of HBM peak on the two-pass byte model of the reference's formulation (element pass:
+    # u, v, h, A, sigma r+w; node pass: sigma and 13 node fields r, u, v w), 8 (9 S + 2 D) per
+    # element plus 8 * 17 per node, FP64: the figure whole-step throughput is judged by.
+    ne, nn = nx * ny, (2 * nx + 1) * (2 * ny + 1)
+    ref_bytes = 8 * (9 * 8 + 2 * 6) * ne + 8 * 17 * nn
     out = {
-        "metric": "Mcell-mEVP-iters/s",
+        "metric": "Mcell-mEVP-iters/s, % HBM peak; learned vs plain speedup at equal residual",
         "value": round(value, 3),
         "unit": "Mcell-mEVP-iters/s",
         "n_gpus": 1,
@@ -225,12 +279,18 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 5)},
         "kernels_ms": {n: (round(v[1], 5) if v[1] else None) for n, v in ms.items()},
+        "hbm_peak_frac": round(value * 1e6 / ne * ref_bytes / HBM_PEAK, 4),
+        "hbm_bytes_per_subcycle_reference_model": ref_bytes,
         "subcycle_hbm_frac": round((eb + nb) / ((stats["mevp_element"][1] + stats["mevp_node"][1])
                                                 / stats["mevp_element"][0] * 1e-3) / HBM_PEAK, 4),
     }
     if args.fp32_nx:
         log(f"bench: FP32 variant {args.fp32_nx}^2 ...")
         out["fp32"] = fp32_variant(args.fp32_nx, max(2, args.steps // 2), 1, args.fp32_error_steps)
+    if args.learned_nx:
+        log(f"bench: learned warm start vs plain at {args.learned_nx}^2 ...")
+        out["learned"] = learned_vs_plain(args.learned_nx)
+        out["learned_vs_plain_speedup_at_equal_residual"] = out["learned"].get("speedup_min")
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:

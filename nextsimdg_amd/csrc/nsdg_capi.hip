@@ -580,23 +580,7 @@ int nsdg_dyn_warm_start(nsdg_dyn* d, nsdg_cnn* net, double* r_predicted, double*
 int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, int* done,
     double* r_final)
 {
-    return guard([&] {
-        auto& k = *nonNull(d, "dyn")->k;
-        require(max_subcycles >= 0 && check_every >= 1, "need max_subcycles >= 0 and check_every >= 1");
-        double r = 0.0;
-        k.residual(&r, nullptr, nullptr);
-        int n = 0;
-        while (r > r_target && n < max_subcycles) {
-            const int step = std::min(check_every, max_subcycles - n);
-            k.subcycles(step);
-            n += step;
-            k.residual(&r, nullptr, nullptr);
-        }
-        if (done)
-            *done = n;
-        if (r_final)
-            *r_final = r;
-    });
+    return guard([&] { nonNull(d, "dyn")->k->subcyclesToResidual(r_target, max_subcycles, check_every, done, r_final); });
 }
 
 } // extern "C"

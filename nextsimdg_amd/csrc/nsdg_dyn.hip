@@ -22,6 +22,7 @@
 #include "nsdg_device.hpp"
 #include "nsdg_dyn.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -1910,25 +1911,64 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         k_node_coeffs<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), fzOcn.p, fzCf.p, fzQ.p, fzIML.p);
         checkLaunch("node_coeffs");
     }
-    void subcyclesAsync(int n)
+    // A run of subcycles: the frozen terms once, the FP32 copy of the stress on entry and
+    // back on exit, the passes in between.
+    void beginRun()
     {
-        if (n <= 0)
-            return;
         frozenAsync();
-        const bool f32 = precision == NSDG_PREC_FP32;
-        if (f32) {
+        if (precision == NSDG_PREC_FP32) {
             k_cast<double, float><<<blocksFor(sig.n), kBlock, 0, s>>>(sig.p, sigF.p, sig.n);
             checkLaunch("stress_to_fp32");
         }
+    }
+    void runPasses(int n)
+    {
         for (int k = 0; k < n; ++k) {
             elementPass();
             nodePass();
             ++subcycleCount;
         }
-        if (f32) {
+    }
+    void endRun()
+    {
+        if (precision == NSDG_PREC_FP32) {
             k_cast<float, double><<<blocksFor(sig.n), kBlock, 0, s>>>(sigF.p, sig.p, sig.n);
             checkLaunch("stress_to_fp64");
         }
+    }
+    void subcyclesAsync(int n)
+    {
+        if (n <= 0)
+            return;
+        beginRun();
+        runPasses(n);
+        endRun();
+    }
+    // Subcycles in chunks until the momentum residual (which reads only h, A, u, v and the
+    // forcing) is at most rTarget or maxN have run.
+    void subcyclesToResidual(double rTarget, int maxN, int every, int* done, double* rFinal) override
+    {
+        need();
+        require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
+        require(maxN >= 0 && every >= 1, "need max_subcycles >= 0 and check_every >= 1");
+        double r = 0.0;
+        residual(&r, nullptr, nullptr);
+        int n = 0;
+        if (r > rTarget && maxN > 0) {
+            beginRun();
+            while (r > rTarget && n < maxN) {
+                const int step = std::min(every, maxN - n);
+                runPasses(step);
+                n += step;
+                residual(&r, nullptr, nullptr);
+            }
+            endRun();
+            synchronize();
+        }
+        if (done)
+            *done = n;
+        if (rFinal)
+            *rFinal = r;
     }
 
     void update(double dt) override
