@@ -127,7 +127,6 @@ def learned_vs_plain(nx, kinds=("heterogeneous", "leads", "plain"), seed=5000, s
     """Speed at equal momentum residual: plain FP64 mEVP (100 subcycles from the cold start)
     against the FP32 path started by the trained network with the residual gate, iterating
     until r <= r(plain). Both from the same saved state at the start of a timestep."""
-    import torch
     from nextsimdg_amd import learned as L
 
     if not os.path.exists(WEIGHTS):
@@ -146,7 +145,7 @@ def learned_vs_plain(nx, kinds=("heterogeneous", "leads", "plain"), seed=5000, s
         r_cold = k.momentum_residual()[0]
         k.subcycles(100)  # warm
         k.load_state(blob)
-        torch.cuda.synchronize()
+        k.synchronize()
         t = time.perf_counter()
         k.subcycles(100)
         k.synchronize()
@@ -202,16 +201,16 @@ def cpu_baseline(nx, ny, seed=0, target_s=12.0):
 
 
 def calibrate_copy(mib=1024, reps=3):
-    """A plain device-to-device copy of known size (torch, plumbing only), for scaling the
-    FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs to bytes."""
-    import torch
+    """A plain device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of rocprofv3 --pmc runs to bytes."""
+    import numpy as np
+    from nextsimdg_amd.learned import DeviceArray
 
     n = mib * 1024 * 1024 // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
+    a = DeviceArray.from_numpy(np.ones((1, n)))
+    b = DeviceArray((1, n), np.float64)
     for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
+        a.gather([0], b)
     print(json.dumps({"copy_bytes_each_way": n * 8, "reps": reps}), flush=True)
 
 

@@ -54,6 +54,14 @@ int nsdg_cnn_adam(nsdg_cnn* net, float lr);
 int nsdg_cnn_set_profiling(nsdg_cnn* net, int on);
 int nsdg_cnn_kernel_stats(nsdg_cnn* net, const char* kernel, int* count, double* total_ms);
 
+/* ---- device buffers for the training data (plain device memory of this library) --- */
+int nsdg_device_alloc(size_t bytes, void** ptr);
+int nsdg_device_free(void* ptr);
+int nsdg_device_upload(void* dst, const void* src, size_t bytes);
+int nsdg_device_download(void* dst, const void* src, size_t bytes);
+/* dst item i <- src item index[i] (items of item_bytes; index on the host), for batches. */
+int nsdg_device_gather(void* dst, const void* src, const int64_t* index, int n, size_t item_bytes);
+
 /* ---- coupling to the dynamics kernel (element grid nx x ny of an initialised CG2 kernel,
  * after begin_iteration) ---------------------------------------------------------------
  * Network input for the current state: per element h and A (cell means), the ocean

@@ -91,6 +91,11 @@ SIGNATURES = {
     "nsdg_cnn_adam": (_i, [_vp, ctypes.c_float]),
     "nsdg_cnn_set_profiling": (_i, [_vp, _i]),
     "nsdg_cnn_kernel_stats": (_i, [_vp, _s, ctypes.POINTER(_i), _dp]),
+    "nsdg_device_alloc": (_i, [_sz, ctypes.POINTER(_vp)]),
+    "nsdg_device_free": (_i, [_vp]),
+    "nsdg_device_upload": (_i, [_vp, _vp, _sz]),
+    "nsdg_device_download": (_i, [_vp, _vp, _sz]),
+    "nsdg_device_gather": (_i, [_vp, _vp, _vp, _i, _sz]),
     "nsdg_dyn_features": (_i, [_vp, _vp]),
     "nsdg_dyn_velocity_target": (_i, [_vp, _vp, _vp]),
     "nsdg_dyn_apply_velocity": (_i, [_vp, _vp]),

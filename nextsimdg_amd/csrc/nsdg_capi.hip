@@ -535,6 +535,55 @@ int nsdg_cnn_kernel_stats(nsdg_cnn* net, const char* kernel, int* count, double*
     });
 }
 
+int nsdg_device_alloc(size_t bytes, void** ptr)
+{
+    return guard([&] {
+        needDevice();
+        nonNull(ptr, "ptr");
+        *ptr = nullptr;
+        if (bytes)
+            NSDG_HIP(hipMalloc(ptr, bytes));
+    });
+}
+int nsdg_device_free(void* ptr)
+{
+    return guard([&] {
+        if (ptr)
+            NSDG_HIP(hipFree(ptr));
+    });
+}
+int nsdg_device_upload(void* dst, const void* src, size_t bytes)
+{
+    return guard([&] {
+        if (bytes)
+            NSDG_HIP(hipMemcpy(nonNull(dst, "dst"), nonNull(src, "src"), bytes, hipMemcpyHostToDevice));
+    });
+}
+int nsdg_device_download(void* dst, const void* src, size_t bytes)
+{
+    return guard([&] {
+        if (bytes)
+            NSDG_HIP(hipMemcpy(nonNull(dst, "dst"), nonNull(src, "src"), bytes, hipMemcpyDeviceToHost));
+    });
+}
+int nsdg_device_gather(void* dst, const void* src, const int64_t* index, int n, size_t item_bytes)
+{
+    return guard([&] {
+        nonNull(dst, "dst");
+        nonNull(src, "src");
+        require(n >= 0, "n must be >= 0");
+        if (n > 0)
+            nonNull(index, "index");
+        for (int i = 0; i < n; ++i) {
+            require(index[i] >= 0, "negative gather index");
+            NSDG_HIP(hipMemcpyAsync(static_cast<char*>(dst) + static_cast<size_t>(i) * item_bytes,
+                static_cast<const char*>(src) + static_cast<size_t>(index[i]) * item_bytes, item_bytes,
+                hipMemcpyDeviceToDevice, nullptr));
+        }
+        NSDG_HIP(hipStreamSynchronize(nullptr));
+    });
+}
+
 int nsdg_dyn_features(nsdg_dyn* d, uint16_t* x)
 {
     return guard([&] { nonNull(d, "dyn")->k->features(nonNull(x, "x")); });

