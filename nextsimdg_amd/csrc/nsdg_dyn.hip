@@ -611,8 +611,8 @@ template <> __device__ __forceinline__ const float* strengthOf<float>(const Elem
 // which the gradient needs, are formed exactly in FP64 before rounding to FP32).
 // VP: sigma <- Pi sigma_VP(eps(u)) outright (no relaxation, the old stress is not read);
 // used for the momentum residual and for the stress of a predicted velocity.
-template <int CG, int DG, class R, class C = R, bool VP = false>
-__device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
+template <int CG, int DG, class R, class C, bool VP>
+__device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
@@ -650,7 +650,6 @@ __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
     // the reference's form. FP32: sigma - sigma/alpha with the same rounded 1/alpha that
     // scales the new term, so that the fixed point is the projected sigma_VP exactly (a
     // separately rounded 1 - 1/alpha would move it by up to 2^-24 alpha relative).
-    Acc<C> acc1[S], acc2[S], acc3[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         if constexpr (VP) {
@@ -744,6 +743,15 @@ __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
             }
         }
     }
+}
+template <int CG, int DG, class R, class C = R, bool VP = false>
+__device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
+{
+    constexpr int S = MomLayout<CG>::S;
+    const size_t ne = a.ne;
+    Acc<C> acc1[S], acc2[S], acc3[S];
+    elementAffineCompute<CG, DG, R, C, VP>(a, e, acc1, acc2, acc3);
+    R* const sig = stressOf<R>(a);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         sig[(0 * S + s) * ne + e] = storeValue<R>(acc1[s], e, 0 * S + s, a.iter);
@@ -918,16 +926,12 @@ __device__ __forceinline__ void divergenceLocal(const NodeArgs& a, int e, double
 // factorisation: stress at the Gauss points, the Jacobian-weighted fluxes
 // T = w |J| J^-T-rows applied to (s11, s12) and (s12, s22), then
 // tx_j = sum_q (PHI_xi[j](q) Tx_xi(q) + PHI_eta[j](q) Tx_eta(q)), factorised per Gauss row.
-// Src: the stored stress type (a.sig for double, a.sigF for float); R: the arithmetic.
-template <int CG, class R, class Src>
-__device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, R* tx, R* ty)
+// Contributions from the stress coefficients of one element (registers).
+template <int CG, class R>
+__device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R* s11, const R* s12, const R* s22,
+    R* tx, R* ty)
 {
     constexpr int S = cg2dgstress(CG), NG = gp1d(S), N = cgdofs(CG), K = CG + 1;
-    R s11[S], s12[S], s22[S];
-    if constexpr (sizeof(Src) == 8)
-        loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
-    else
-        loadTensor<S>(a.sigF, a.ne, e, s11, s12, s22);
     const R xiX = static_cast<R>(a.xiX), etaX = static_cast<R>(a.etaX);
     const R xiY = static_cast<R>(a.xiY), etaY = static_cast<R>(a.etaY);
     Acc<R> sx[N], sy[N];
@@ -988,6 +992,18 @@ __device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, R* tx
         ty[j] = sy[j].get();
     }
 }
+// Src: the stored stress type (a.sig for double, a.sigF for float); R: the arithmetic.
+template <int CG, class R, class Src>
+__device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, R* tx, R* ty)
+{
+    constexpr int S = cg2dgstress(CG);
+    R s11[S], s12[S], s22[S];
+    if constexpr (sizeof(Src) == 8)
+        loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
+    else
+        loadTensor<S>(a.sigF, a.ne, e, s11, s12, s22);
+    divergenceAffineFrom<CG, R>(a, s11, s12, s22, tx, ty);
+}
 
 // The momentum update with the terms that stay fixed over the subcycles of one timestep
 // (u0, v0, wind, H, A, SSH gradient, lumped mass) folded once per timestep into
@@ -1034,15 +1050,73 @@ __device__ __forceinline__ void momentumFrozen(const NodeArgs& a, const NodeFz& 
     vn = inv * (c1 * a.beta * in.v + in.qf.y + drag * in.ocn.y + cor * in.v + dSy * in.iML);
 }
 
-template <int CG> struct NodeTile {
-    // 64 x 4 elements: long rows keep the 128-byte lines of the stress and node arrays
-    // well used; the tile plus its left/lower halo is 65 x 5 elements.
-    static constexpr int TX = 64, TY = 4;
+// Node tiles: a block owns the CG nodes of the lower-left 2x2 (CG2) node blocks of TX x TY
+// elements (the last tile row/column also owns the domain edge) and gathers the weak
+// divergence of the tile and a one-element left/bottom halo (HX x HY elements) from LDS.
+template <int CG, int TX_, int TY_> struct TileShape {
+    static constexpr int TX = TX_, TY = TY_;
     static constexpr int HX = TX + 1, HY = TY + 1, N = cgdofs(CG);
     static constexpr int CW = CG * TX;                        // owned node columns
     static constexpr int TPR = CW < kBlock ? CW : kBlock;     // threads per node row
     static constexpr int RS = kBlock / TPR;                   // node rows per block sweep
+    static constexpr int NB = (CG * TY + RS - 1) / RS;        // nodes per thread per sweep
 };
+// Node pass: 64 x 4 elements, long rows for the 128-byte lines of the stress and node arrays.
+template <int CG> using NodeTile = TileShape<CG, 64, 4>;
+// Fused subcycle: 31 x 7 elements, so that tile and halo are exactly one element per thread.
+template <int CG> using FusedTile = TileShape<CG, 31, 7>;
+static_assert(FusedTile<2>::HX * FusedTile<2>::HY == kBlock, "fused tile must give one element per thread");
+
+// Phase 2 of a node tile: the owned nodes sum their <= 4 contributions (two-colour stripe
+// order, land contributions exact zeros) and take the FP64 momentum update from the frozen
+// coefficients; u, v are read from a.u, a.v and written to uOut, vOut. Each thread updates
+// up to NB nodes of one column, all their loads issued first.
+template <int CG, class T, class R>
+__device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const R (*contrib)[cgdofs(CG)][2], int ex0, int ey0,
+    double* uOut, double* vOut)
+{
+    const int row = CG * a.nx + 1;
+    const int nx0 = CG * ex0, ny0 = CG * ey0;
+    const int W = (ex0 + T::TX >= a.nx) ? CG * a.nx + 1 - nx0 : T::CW;
+    const int H = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 - ny0 : CG * T::TY;
+    constexpr int NB = T::NB;
+    if (threadIdx.x >= T::TPR * T::RS)
+        return;
+    for (int c = threadIdx.x % T::TPR; c < W; c += T::TPR) {
+        for (int r0 = threadIdx.x / T::TPR; r0 < H; r0 += NB * T::RS) {
+            NodeFz in[NB];
+            bool live[NB], dir[NB];
+            const int nix = nx0 + c;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int r = r0 + b * T::RS;
+                live[b] = r < H;
+                const int n = nix + row * (ny0 + r);
+                dir[b] = live[b] ? a.dirNode[n] != 0 : true;
+                if (live[b] && !dir[b])
+                    loadFrozen(a, n, in[b]);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (!live[b])
+                    continue;
+                const int niy = ny0 + r0 + b * T::RS;
+                double un = 0.0, vn = 0.0;
+                if (!dir[b]) {
+                    double dx = 0.0, dy = 0.0;
+                    forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
+                        const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
+                        dx -= static_cast<double>(contrib[le][j][0]);
+                        dy -= static_cast<double>(contrib[le][j][1]);
+                    });
+                    momentumFrozen(a, in[b], dx, dy, un, vn);
+                }
+                uOut[nix + row * niy] = un;
+                vOut[nix + row * niy] = vn;
+            }
+        }
+    }
+}
 
 // Fused stressDivergence + updateMomentum + applyBoundaries (no periodic segments).
 template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
@@ -1080,49 +1154,68 @@ template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k
         }
     }
     __syncthreads();
+    // Phase 2
+    nodeTileUpdate<CG, T, R>(a, contrib, ex0, ey0, a.u, a.v);
+}
 
-    // Phase 2: nodes owned by the tile (the last tile row/column also owns the domain edge);
-    // each thread updates up to NB nodes of one column, all their loads issued first.
-    const int row = CG * a.nx + 1;
-    const int nx0 = CG * ex0, ny0 = CG * ey0;
-    const int W = (ex0 + T::TX >= a.nx) ? CG * a.nx + 1 - nx0 : T::CW;
-    const int H = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 - ny0 : CG * T::TY;
-    constexpr int NB = CG * T::TY / T::RS;
-    for (int c = threadIdx.x % T::TPR; c < W; c += T::TPR) {
-        for (int r0 = threadIdx.x / T::TPR; r0 < H; r0 += NB * T::RS) {
-            NodeFz in[NB];
-            bool live[NB], dir[NB];
-            const int nix = nx0 + c;
+// One whole mEVP subcycle in one pass over a uniform mesh (FP64): per tile, phase 1 runs
+// the element update (strain, stress, projection) of every element of the tile and its
+// left/bottom halo, keeps the new stress in registers, stores it for the tile's own
+// elements only (halo elements belong to other tiles, which compute them identically),
+// and turns it into weak-divergence contributions in LDS; phase 2 is the node update.
+// Stress and velocities are ping-ponged: the pass reads sigma^k, u^k, v^k (e.sig, n.u,
+// n.v) and writes sigma^(k+1), u^(k+1), v^(k+1) to other buffers, so no tile reads a
+// neighbour's new values. The arithmetic is that of k_mevp_element then k_mevp_node.
+struct FusedArgs {
+    ElemArgs e;
+    NodeArgs n;
+    double* sigOut;
+    double* uOut;
+    double* vOut;
+};
+template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_fused(FusedArgs f)
+{
+    using T = FusedTile<CG>;
+    constexpr int N = T::N, S = cg2dgstress(CG);
+    __shared__ double contrib[T::HX * T::HY][N][2];
+    const NodeArgs& a = f.n;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+    {
+        const int le = threadIdx.x; // one element of tile + halo per thread
+        const int lx = le % T::HX, ly = le / T::HX;
+        const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
+        double tx[N], ty[N];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int r = r0 + b * T::RS;
-                live[b] = r < H;
-                const int n = nix + row * (ny0 + r);
-                dir[b] = live[b] ? a.dirNode[n] != 0 : true;
-                if (live[b] && !dir[b])
-                    loadFrozen(a, n, in[b]);
-            }
+        for (int j = 0; j < N; ++j)
+            tx[j] = ty[j] = 0.0;
+        if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
+            const int e = ex + a.nx * ey;
+            Acc<double> acc1[S], acc2[S], acc3[S];
+            elementAffineCompute<CG, DG, double, double, false>(f.e, e, acc1, acc2, acc3);
+            double s11[S], s12[S], s22[S];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (!live[b])
-                    continue;
-                const int niy = ny0 + r0 + b * T::RS;
-                double un = 0.0, vn = 0.0;
-                if (!dir[b]) {
-                    // land contributions are stored as exact zeros: subtracting them changes nothing
-                    double dx = 0.0, dy = 0.0;
-                    forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
-                        const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
-                        dx -= static_cast<double>(contrib[le][j][0]);
-                        dy -= static_cast<double>(contrib[le][j][1]);
-                    });
-                    momentumFrozen(a, in[b], dx, dy, un, vn);
-                }
-                a.u[nix + row * niy] = un;
-                a.v[nix + row * niy] = vn;
+            for (int s = 0; s < S; ++s) {
+                s11[s] = acc1[s].get();
+                s12[s] = acc2[s].get();
+                s22[s] = acc3[s].get();
             }
+            if (lx >= 1 && ly >= 1)
+                storeTensor<S>(f.sigOut, a.ne, e, s11, s12, s22);
+            if (a.ocean[e])
+                divergenceAffineFrom<CG, double>(a, s11, s12, s22, tx, ty);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            contrib[le][j][0] = tx[j];
+            contrib[le][j][1] = ty[j];
         }
     }
+    __syncthreads();
+    nodeTileUpdate<CG, T, double>(a, contrib, ex0, ey0, f.uOut, f.vOut);
 }
 
 // Split passes (periodic meshes and per-pass testing).
@@ -1470,6 +1563,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
     DBuf<double> sigVP, rpart, rsum; // momentum residual: VP stress, block partials, sums
     DBuf<double> uStart, vStart, sigStart; // saved start of a timestep (warm-start gate)
+    DBuf<double> uAlt, vAlt, sigAlt;       // ping-pong partners of u, v, sigma (fused subcycle)
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
 
     explicit DynImpl(const nsdg_params& prm)
@@ -1528,6 +1622,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         res.alloc(2);
         sigVP.alloc(3 * S * ne);
         deviceFill(sigVP.p, sigVP.n, 0.0, s);
+        uAlt.alloc(nn);
+        vAlt.alloc(nn);
+        sigAlt.alloc(3 * S * ne);
         rpart.alloc(2 * kResidualBlocks);
         rsum.alloc(2);
         for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
@@ -1921,11 +2018,40 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             checkLaunch("stress_to_fp32");
         }
     }
+    // One-pass subcycles: uniform meshes without periodic segments, FP64.
+    bool fusedSubcycle() const { return mesh.uniform && !mesh.hasPeriodic && precision == NSDG_PREC_FP64; }
+    static void swapBuf(DBuf<double>& x, DBuf<double>& y)
+    {
+        std::swap(x.p, y.p);
+        std::swap(x.n, y.n);
+    }
+    void fusedPass()
+    {
+        Timed t(prof, s, "mevp_fused");
+        FusedArgs f {};
+        f.e = elemArgs();
+        f.n = nodeArgs();
+        f.sigOut = sigAlt.p;
+        f.uOut = uAlt.p;
+        f.vOut = vAlt.p;
+        using T = FusedTile<CG>;
+        const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+        k_mevp_fused<CG, DG><<<grid, kBlock, 0, s>>>(f);
+        checkLaunch("mevp_fused");
+        // the new state becomes the current one; the old buffers are the next pass's outputs
+        swapBuf(sig, sigAlt);
+        swapBuf(u, uAlt);
+        swapBuf(v, vAlt);
+    }
     void runPasses(int n)
     {
         for (int k = 0; k < n; ++k) {
-            elementPass();
-            nodePass();
+            if (fusedSubcycle()) {
+                fusedPass();
+            } else {
+                elementPass();
+                nodePass();
+            }
             ++subcycleCount;
         }
     }
