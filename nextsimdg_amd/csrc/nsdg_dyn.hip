@@ -334,7 +334,7 @@ template <int S, class T, class U> __device__ __forceinline__ void loadTensor(co
         c[s] = f[(2 * S + s) * ne + e];
     }
 }
-template <int S> __device__ __forceinline__ void storeTensor(double* f, size_t ne, int e, const double* a, const double* b, const double* c)
+template <int S, class T> __device__ __forceinline__ void storeTensor(T* f, size_t ne, int e, const T* a, const T* b, const T* c)
 {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1166,18 +1166,21 @@ template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k
 // Stress and velocities are ping-ponged: the pass reads sigma^k, u^k, v^k (e.sig, n.u,
 // n.v) and writes sigma^(k+1), u^(k+1), v^(k+1) to other buffers, so no tile reads a
 // neighbour's new values. The arithmetic is that of k_mevp_element then k_mevp_node.
+// R = float: the FP32 variant (stress stored as written by the two-pass FP32 kernels,
+// stochastically rounded; the divergence uses the stored values).
 struct FusedArgs {
     ElemArgs e;
     NodeArgs n;
     double* sigOut;
+    float* sigFOut;
     double* uOut;
     double* vOut;
 };
-template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_fused(FusedArgs f)
+template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_mevp_fused(FusedArgs f)
 {
     using T = FusedTile<CG>;
     constexpr int N = T::N, S = cg2dgstress(CG);
-    __shared__ double contrib[T::HX * T::HY][N][2];
+    __shared__ R contrib[T::HX * T::HY][N][2];
     const NodeArgs& a = f.n;
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
     const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
@@ -1188,25 +1191,29 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_fused
         const int le = threadIdx.x; // one element of tile + halo per thread
         const int lx = le % T::HX, ly = le / T::HX;
         const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
-        double tx[N], ty[N];
+        R tx[N], ty[N];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            tx[j] = ty[j] = 0.0;
+            tx[j] = ty[j] = 0;
         if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
             const int e = ex + a.nx * ey;
-            Acc<double> acc1[S], acc2[S], acc3[S];
-            elementAffineCompute<CG, DG, double, double, false>(f.e, e, acc1, acc2, acc3);
-            double s11[S], s12[S], s22[S];
+            Acc<R> acc1[S], acc2[S], acc3[S];
+            elementAffineCompute<CG, DG, R, R, false>(f.e, e, acc1, acc2, acc3);
+            R s11[S], s12[S], s22[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                s11[s] = acc1[s].get();
-                s12[s] = acc2[s].get();
-                s22[s] = acc3[s].get();
+                s11[s] = storeValue<R>(acc1[s], e, 0 * S + s, f.e.iter);
+                s12[s] = storeValue<R>(acc2[s], e, 1 * S + s, f.e.iter);
+                s22[s] = storeValue<R>(acc3[s], e, 2 * S + s, f.e.iter);
             }
-            if (lx >= 1 && ly >= 1)
-                storeTensor<S>(f.sigOut, a.ne, e, s11, s12, s22);
+            if (lx >= 1 && ly >= 1) {
+                if constexpr (sizeof(R) == 8)
+                    storeTensor<S>(f.sigOut, a.ne, e, s11, s12, s22);
+                else
+                    storeTensor<S>(f.sigFOut, a.ne, e, s11, s12, s22);
+            }
             if (a.ocean[e])
-                divergenceAffineFrom<CG, double>(a, s11, s12, s22, tx, ty);
+                divergenceAffineFrom<CG, R>(a, s11, s12, s22, tx, ty);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -1215,7 +1222,7 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_fused
         }
     }
     __syncthreads();
-    nodeTileUpdate<CG, T, double>(a, contrib, ex0, ey0, f.uOut, f.vOut);
+    nodeTileUpdate<CG, T, R>(a, contrib, ex0, ey0, f.uOut, f.vOut);
 }
 
 // Split passes (periodic meshes and per-pass testing).
@@ -1560,6 +1567,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<double2> fzOcn, fzCf, fzQ; // frozen momentum coefficients (k_node_coeffs)
     DBuf<double> fzIML, strength;   // 1 / ML; ice strength at the Gauss points
     DBuf<float> sigF, strengthF;    // FP32 variant: stress and strength
+    DBuf<float> sigFAlt;            // ping-pong partner of sigF (fused subcycle)
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
     DBuf<double> sigVP, rpart, rsum; // momentum residual: VP stress, block partials, sums
     DBuf<double> uStart, vStart, sigStart; // saved start of a timestep (warm-start gate)
@@ -2018,9 +2026,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             checkLaunch("stress_to_fp32");
         }
     }
-    // One-pass subcycles: uniform meshes without periodic segments, FP64.
-    bool fusedSubcycle() const { return mesh.uniform && !mesh.hasPeriodic && precision == NSDG_PREC_FP64; }
-    static void swapBuf(DBuf<double>& x, DBuf<double>& y)
+    // One-pass subcycles: uniform meshes without periodic segments.
+    bool fusedSubcycle() const { return mesh.uniform && !mesh.hasPeriodic; }
+    template <class X> static void swapBuf(DBuf<X>& x, DBuf<X>& y)
     {
         std::swap(x.p, y.p);
         std::swap(x.n, y.n);
@@ -2032,14 +2040,23 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         f.e = elemArgs();
         f.n = nodeArgs();
         f.sigOut = sigAlt.p;
+        f.sigFOut = sigFAlt.p;
         f.uOut = uAlt.p;
         f.vOut = vAlt.p;
         using T = FusedTile<CG>;
         const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-        k_mevp_fused<CG, DG><<<grid, kBlock, 0, s>>>(f);
+        if (precision == NSDG_PREC_FP32) {
+            require(sigF.n == 3 * S * ne && sigFAlt.n == sigF.n, "FP32 buffers not allocated", NSDG_ERR_STATE);
+            k_mevp_fused<CG, DG, float><<<grid, kBlock, 0, s>>>(f);
+        } else {
+            k_mevp_fused<CG, DG, double><<<grid, kBlock, 0, s>>>(f);
+        }
         checkLaunch("mevp_fused");
         // the new state becomes the current one; the old buffers are the next pass's outputs
-        swapBuf(sig, sigAlt);
+        if (precision == NSDG_PREC_FP32)
+            swapBuf(sigF, sigFAlt);
+        else
+            swapBuf(sig, sigAlt);
         swapBuf(u, uAlt);
         swapBuf(v, vAlt);
     }
@@ -2302,6 +2319,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     {
         if (precision == NSDG_PREC_FP32 && ready && sigF.n != 3 * S * ne) {
             sigF.alloc(3 * S * ne);
+            sigFAlt.alloc(3 * S * ne);
             strengthF.alloc(static_cast<size_t>(L::Q) * ne);
         }
     }
