@@ -646,24 +646,20 @@ __device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, A
     const C xiX = static_cast<C>(a.xiX), etaX = static_cast<C>(a.etaX);
     const C xiY = static_cast<C>(a.xiY), etaY = static_cast<C>(a.etaY);
     const C DM2 = static_cast<C>(a.DM2);
-    // previous stress, relaxed in place. FP64: the accumulators start at (1 - 1/alpha) sigma,
-    // the reference's form. FP32: sigma - sigma/alpha with the same rounded 1/alpha that
-    // scales the new term, so that the fixed point is the projected sigma_VP exactly (a
-    // separately rounded 1 - 1/alpha would move it by up to 2^-24 alpha relative).
+    // The relaxed previous stress. FP64: the projected terms are summed first and
+    // (1 - 1/alpha) sigma is added at the end (the order of MEVPStressUpdateStep.hpp:78-116);
+    // the old stress is read only then, which keeps 3 S registers free through the strain.
+    // FP32: sigma - sigma/alpha with the same rounded 1/alpha that scales the new term, so
+    // that the fixed point is the projected sigma_VP exactly (a separately rounded
+    // 1 - 1/alpha would move it by up to 2^-24 alpha relative).
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if constexpr (VP) {
+        if constexpr (VP || sizeof(C) == 8) {
             acc1[s].init(0);
             acc2[s].init(0);
             acc3[s].init(0);
-            continue;
-        }
-        const C o1 = sig[(0 * S + s) * ne + e], o2 = sig[(1 * S + s) * ne + e], o3 = sig[(2 * S + s) * ne + e];
-        if constexpr (sizeof(C) == 8) {
-            acc1[s].initProduct(o1, a.relax);
-            acc2[s].initProduct(o2, a.relax);
-            acc3[s].initProduct(o3, a.relax);
         } else {
+            const C o1 = sig[(0 * S + s) * ne + e], o2 = sig[(1 * S + s) * ne + e], o3 = sig[(2 * S + s) * ne + e];
             const C mInvAlpha = -static_cast<C>(a.invAlpha);
             acc1[s].init(o1);
             acc2[s].init(o2);
@@ -741,6 +737,14 @@ __device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, A
                 acc2[s].add(static_cast<C>(bs) * r12);
                 acc3[s].add(static_cast<C>(bs) * r22);
             }
+        }
+    }
+    if constexpr (sizeof(C) == 8 && !VP) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            acc1[s].s = sig[(0 * S + s) * ne + e] * a.relax + acc1[s].s;
+            acc2[s].s = sig[(1 * S + s) * ne + e] * a.relax + acc2[s].s;
+            acc3[s].s = sig[(2 * S + s) * ne + e] * a.relax + acc3[s].s;
         }
     }
 }
