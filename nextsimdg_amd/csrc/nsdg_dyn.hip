@@ -783,14 +783,6 @@ template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_
         storeTensor<S>(a.sig, a.ne, e, s11, s12, s22);
     }
 }
-// FP32 variant of the fused pass (uniform meshes).
-template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_mevp_element_f32(ElemArgs a)
-{
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.ne)
-        return;
-    elementAffine<CG, DG, float, float>(a, e);
-}
 // sigma <- Pi sigma_VP(eps(u)) with a.invAlpha = 1 (the converged mEVP stress of u).
 template <int CG, int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_vp_stress(ElemArgs a)
 {
@@ -838,7 +830,6 @@ struct NodeArgs {
     const uint8_t* ocean;
     const uint8_t* dirNode;
     const double* sig;
-    const float* sigF; // FP32 variant
     const double* ops;
     size_t stride;
     double* u;
@@ -996,18 +987,6 @@ __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R*
         ty[j] = sy[j].get();
     }
 }
-// Src: the stored stress type (a.sig for double, a.sigF for float); R: the arithmetic.
-template <int CG, class R, class Src>
-__device__ __forceinline__ void divergenceAffine(const NodeArgs& a, int e, R* tx, R* ty)
-{
-    constexpr int S = cg2dgstress(CG);
-    R s11[S], s12[S], s22[S];
-    if constexpr (sizeof(Src) == 8)
-        loadTensor<S>(a.sig, a.ne, e, s11, s12, s22);
-    else
-        loadTensor<S>(a.sigF, a.ne, e, s11, s12, s22);
-    divergenceAffineFrom<CG, R>(a, s11, s12, s22, tx, ty);
-}
 
 // The momentum update with the terms that stay fixed over the subcycles of one timestep
 // (u0, v0, wind, H, A, SSH gradient, lumped mass) folded once per timestep into
@@ -1122,13 +1101,13 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const R (*cont
     }
 }
 
-// Fused stressDivergence + updateMomentum + applyBoundaries (no periodic segments).
-template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
+// Fused stressDivergence + updateMomentum + applyBoundaries on a general mesh without
+// periodic segments (per-element operators; uniform meshes run k_mevp_fused instead).
+template <int CG> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
 {
-    static_assert(UNI || sizeof(R) == 8, "the FP32 node pass needs a uniform mesh");
     using T = NodeTile<CG>;
     constexpr int N = T::N;
-    __shared__ R contrib[T::HX * T::HY][N][2];
+    __shared__ double contrib[T::HX * T::HY][N][2];
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
     const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
     if (tile < 0)
@@ -1138,18 +1117,14 @@ template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k
     // Phase 1: weak-divergence contributions of the tile and its left/bottom halo.
     for (int le = threadIdx.x; le < T::HX * T::HY; le += kBlock) {
         const int ex = ex0 - 1 + le % T::HX, ey = ey0 - 1 + le / T::HX;
-        R tx[N], ty[N];
+        double tx[N], ty[N];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            tx[j] = ty[j] = 0;
+            tx[j] = ty[j] = 0.0;
         if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
             const int e = ex + a.nx * ey;
-            if (a.ocean[e]) {
-                if constexpr (UNI)
-                    divergenceAffine<CG, R, R>(a, e, tx, ty);
-                else
-                    divergenceLocal<CG, false>(a, e, tx, ty);
-            }
+            if (a.ocean[e])
+                divergenceLocal<CG, false>(a, e, tx, ty);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -1159,7 +1134,7 @@ template <int CG, bool UNI, class R> __global__ __launch_bounds__(kBlock) void k
     }
     __syncthreads();
     // Phase 2
-    nodeTileUpdate<CG, T, R>(a, contrib, ex0, ey0, a.u, a.v);
+    nodeTileUpdate<CG, T, double>(a, contrib, ex0, ey0, a.u, a.v);
 }
 
 // One whole mEVP subcycle in one pass over a uniform mesh (FP64): per tile, phase 1 runs
@@ -1901,7 +1876,6 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.ocean = mesh.ocean.p;
         a.dirNode = dirNode.p;
         a.sig = sig.p;
-        a.sigF = sigF.p;
         a.ops = mops.p;
         a.stride = opStride();
         a.u = u.p;
@@ -1937,16 +1911,14 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.detJ = mesh.detJ;
         return a;
     }
+    // Two-pass subcycle (general meshes, periodic meshes); FP64 only, the FP32 variant always
+    // runs the one-pass kernel.
     void elementPass()
     {
+        require(precision == NSDG_PREC_FP64, "the two-pass subcycle is FP64", NSDG_ERR_STATE);
         Timed t(prof, s, "mevp_element");
         const ElemArgs a = elemArgs();
-        if (precision == NSDG_PREC_FP32) {
-            require(sigF.n == 3 * S * ne && strengthF.n == static_cast<size_t>(L::Q) * ne,
-                "FP32 buffers not allocated", NSDG_ERR_STATE);
-            k_mevp_element_f32<CG, DG><<<blocksFor(ne), kBlock, 0, s>>>(a);
-        }
-        else if (mesh.uniform)
+        if (mesh.uniform)
             k_mevp_element<CG, DG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
         else
             k_mevp_element<CG, DG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
@@ -1959,15 +1931,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             Timed t(prof, s, "mevp_node");
             using T = NodeTile<CG>;
             const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-            if (precision == NSDG_PREC_FP32) {
-                require(mesh.uniform && sigF.n == 3 * S * ne, "FP32 node pass needs its buffers and a uniform mesh",
-                    NSDG_ERR_STATE);
-                k_mevp_node<CG, true, float><<<grid, kBlock, 0, s>>>(a);
-            }
-            else if (mesh.uniform)
-                k_mevp_node<CG, true, double><<<grid, kBlock, 0, s>>>(a);
-            else
-                k_mevp_node<CG, false, double><<<grid, kBlock, 0, s>>>(a);
+            require(!mesh.uniform, "uniform meshes without periodic segments run the one-pass kernel",
+                NSDG_ERR_STATE);
+            k_mevp_node<CG><<<grid, kBlock, 0, s>>>(a);
             checkLaunch("mevp_node");
             return;
         }
