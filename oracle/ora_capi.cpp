@@ -187,11 +187,15 @@ struct DynBase {
     virtual void beginIteration(double) = 0;
     virtual void pass(int) = 0;
     virtual size_t ncg() = 0;
+    virtual void setRheology(int) = 0;
+    virtual MEBParams& meb() = 0;
 };
 template <int CG, int DG> struct DynT : DynBase {
     Kernel<CG, DG> k;
     Params& params() override { return k.params; }
     Mesh& mesh() override { return k.mesh; }
+    void setRheology(int r) override { k.rheology = r; }
+    MEBParams& meb() override { return k.meb; }
     void initialise(size_t nx, size_t ny, const double* c, const double* m) override { k.initialise(nx, ny, c, m); }
     void finishInit() override { k.finishInit(); }
     void setData(const std::string& n, const double* d, int c) override { k.setData(n, d, c); }
@@ -522,6 +526,31 @@ void* ora_dyn_new(int cg, int dg, const double* p)
         q.alpha = p[12];
         q.beta = p[13];
         q.nSteps = static_cast<int>(p[14]);
+    }
+    return d;
+}
+// The BBM kernel (BBMDynamicsKernel.hpp): as ora_dyn_new, plus the MEB parameters
+// compaction_param, nu0, young, P0, undamaged_time_relaxation_sigma,
+// exponent_relaxation_sigma, exponent_compression_factor, tan_phi, compr_strength, C_lab
+// (NULL: MEBParameters defaults). nSteps in p applies to the brittle substeps.
+void* ora_dyn_new_bbm(int cg, int dg, const double* p, const double* meb)
+{
+    auto* d = static_cast<DynBase*>(ora_dyn_new(cg, dg, p));
+    if (!d)
+        return nullptr;
+    d->setRheology(1);
+    if (meb) {
+        MEBParams& m = d->meb();
+        m.compaction_param = meb[0];
+        m.nu0 = meb[1];
+        m.young = meb[2];
+        m.P0 = meb[3];
+        m.undamaged_time_relaxation_sigma = meb[4];
+        m.exponent_relaxation_sigma = meb[5];
+        m.exponent_compression_factor = meb[6];
+        m.tan_phi = meb[7];
+        m.compr_strength = meb[8];
+        m.C_lab = meb[9];
     }
     return d;
 }

@@ -866,6 +866,24 @@ struct Params {
     int nSteps = 100;
 };
 
+// Brittle rheology parameters: MEBParameters defaults (MEBParameters.hpp:38-70).
+struct MEBParams {
+    double compaction_param = -20.0, nu0 = 1.0 / 3.0, young = 5.96e8, P0 = 10.e3;
+    double undamaged_time_relaxation_sigma = 1e7, exponent_relaxation_sigma = 5.0;
+    double exponent_compression_factor = 1.5, tan_phi = 0.7, compr_strength = 1e10, C_lab = 2.0e6;
+};
+// Element size h = sqrt(area) with the quadrilateral area formula of ParametricMesh::area
+// (ParametricMesh.hpp:276-299): 0.25 sqrt(4 e f - (b + d - a - c)^2) from the squared
+// side lengths a (lower), b (right), c (top), d (left) and diagonals e, f.
+inline double elementSize(const Mesh& m, size_t eid)
+{
+    double c[4][2]; // LL, LR, UL, UR
+    m.coordinatesOfElement(eid, c);
+    auto sq = [&](int i, int j) { return (c[i][0] - c[j][0]) * (c[i][0] - c[j][0]) + (c[i][1] - c[j][1]) * (c[i][1] - c[j][1]); };
+    const double a = sq(0, 1), b = sq(1, 3), cc = sq(2, 3), d = sq(0, 2), e = sq(0, 3), f = sq(1, 2);
+    return std::sqrt(0.25 * std::sqrt(4.0 * e * f - (b + d - a - cc) * (b + d - a - cc)));
+}
+
 // ------------------------------------------------------------------ momentum map
 // ParametricMomentumMap<CG,DG> (dynamics/src/include/ParametricMap.hpp:62-131,
 // dynamics/src/ParametricMap.cpp:94-356), Cartesian
@@ -1032,6 +1050,13 @@ template <int CG, int DG> struct Kernel {
     Mesh mesh;
     Transport<DG>* tr = nullptr;
     MomentumMap<CG>* pmap = nullptr;
+    // Rheology: 0 mEVP (MEVPDynamicsKernel), 1 BBM (BBMDynamicsKernel,
+    // BrittleCGDynamicsKernel.hpp + BBMStressUpdateStep.hpp).
+    int rheology = 0;
+    MEBParams meb;
+    Transport<S>* stressTr = nullptr;
+    std::vector<double> damage, avgU, avgV, hElem;
+    std::vector<std::array<double, DG * Q>> iMJwPSIdam; // [d][q]
     std::vector<double> hice, cice, ssh, e11, e12, e22, s11, s12, s22;
     std::vector<double> u, v, u0, v0, cgH, cgA, gradX, gradY, dSx, dSy, uOcean, vOcean, uAtmos, vAtmos;
     std::map<std::string, std::vector<double>> extra;
@@ -1042,6 +1067,7 @@ template <int CG, int DG> struct Kernel {
     {
         delete tr;
         delete pmap;
+        delete stressTr;
     }
     size_t ncg() const { return (CG * mesh.nx + 1) * (CG * mesh.ny + 1); }
 
@@ -1066,6 +1092,40 @@ template <int CG, int DG> struct Kernel {
             x->assign(ne * S, 0.0);
         for (auto* x : { &u, &v, &u0, &v0, &cgH, &cgA, &gradX, &gradY, &dSx, &dSy, &uOcean, &vOcean, &uAtmos, &vAtmos })
             x->assign(ncg(), 0.0);
+        if (rheology == 1)
+            initBrittle();
+    }
+    // BrittleCGDynamicsKernel::initialise (BrittleCGDynamicsKernel.hpp:72-87) and the damage
+    // projection iMJwPSI_dam (ParametricMap.cpp:290-296). The damage starts undamaged (1).
+    void initBrittle()
+    {
+        const size_t ne = mesh.nelements;
+        delete stressTr;
+        stressTr = new Transport<S>(mesh);
+        stressTr->scheme = 2;
+        damage.assign(ne * DG, 0.0);
+        for (size_t e = 0; e < ne; ++e)
+            damage[e * DG] = 1.0;
+        avgU.assign(ncg(), 0.0);
+        avgV.assign(ncg(), 0.0);
+        hElem.resize(ne);
+        iMJwPSIdam.resize(ne);
+        const auto& P = PsiT<DG, NG>::get();
+        const auto& W = WT<NG>::get();
+        for (size_t e = 0; e < ne; ++e) {
+            hElem[e] = elementSize(mesh, e);
+            double J[Q], M[DG][DG], Mi[DG][DG];
+            jacobian<NG>(mesh, e, J);
+            massMatrix<DG>(mesh, e, M);
+            invert<DG>(M, Mi);
+            for (int d = 0; d < DG; ++d)
+                for (int q = 0; q < Q; ++q) {
+                    double a = 0;
+                    for (int k = 0; k < DG; ++k)
+                        a += Mi[d][k] * P.v[k][q] * (W.w[q] * J[q]);
+                    iMJwPSIdam[e][d * Q + q] = a;
+                }
+        }
     }
 
     // DGModelArray::ma2dg (DGModelArray.hpp:20-32): zero, then all components if the
@@ -1106,6 +1166,8 @@ template <int CG, int DG> struct Kernel {
             ma2dg(data, ncomp, DG, ne, cice);
         else if (name == "ssh")
             ma2dg(data, ncomp, 1, ne, ssh);
+        else if (name == "damage" && rheology == 1)
+            ma2dg(data, ncomp, DG, ne, damage);
         else if (name == "s11" || name == "s12" || name == "s22" || name == "e11" || name == "e12" || name == "e22")
             ma2dg(data, ncomp, S, ne, stressRef(name));
         else {
@@ -1139,6 +1201,8 @@ template <int CG, int DG> struct Kernel {
         if (n == "uwind") return &uAtmos;
         if (n == "vwind") return &vAtmos;
         if (n == "lumpedcgmass") return &pmap->lumped;
+        if (n == "avgU" && rheology == 1) return &avgU;
+        if (n == "avgV" && rheology == 1) return &avgV;
         return nullptr;
     }
 
@@ -1146,6 +1210,18 @@ template <int CG, int DG> struct Kernel {
     std::vector<double> iceOceanStress(bool x) const
     {
         std::vector<double> t(ncg());
+        if (rheology == 1) {
+            // BrittleCGDynamicsKernel::getIceOceanStress (BrittleCGDynamicsKernel.hpp:163-182):
+            // from the averaged velocity, relative velocity ocean minus ice, turned.
+            const double th = params.ocean_turning_angle * 0x1.1df46a2529d39p-6;
+            const double ct = std::cos(th), st = std::sin(th);
+            for (size_t i = 0; i < t.size(); ++i) {
+                const double ur = uOcean[i] - avgU[i], vr = vOcean[i] - avgV[i];
+                const double cp = params.F_ocean * std::hypot(ur, vr);
+                t[i] = x ? cp * (ur * ct - vr * st) : cp * (vr * ct + ur * st);
+            }
+            return t;
+        }
 #pragma omp parallel for
         for (size_t i = 0; i < t.size(); ++i) {
             const double ur = u[i] - uOcean[i], vr = v[i] - vOcean[i];
@@ -1176,6 +1252,8 @@ template <int CG, int DG> struct Kernel {
             src = &hice;
         else if (name == "cice")
             src = &cice;
+        else if (name == "damage" && rheology == 1)
+            src = &damage;
         else if (name == "ssh") {
             src = &ssh;
             D = 1;
@@ -1199,8 +1277,12 @@ template <int CG, int DG> struct Kernel {
     // DynamicsKernel::advectionAndLimits (DynamicsKernel.hpp:160-172)
     void advectionAndLimits(double dt)
     {
-        CG2DG<CG, DG>(mesh, tr->velx.data(), u.data());
-        CG2DG<CG, DG>(mesh, tr->vely.data(), v.data());
+        // the brittle kernels advect with the velocity averaged over the previous step's
+        // substeps (BrittleCGDynamicsKernel.hpp:89-90)
+        const std::vector<double>& ua = rheology == 1 ? avgU : u;
+        const std::vector<double>& va = rheology == 1 ? avgV : v;
+        CG2DG<CG, DG>(mesh, tr->velx.data(), ua.data());
+        CG2DG<CG, DG>(mesh, tr->vely.data(), va.data());
         tr->reinitnormalvelocity();
         tr->step(dt, cice.data());
         tr->step(dt, hice.data());
@@ -1339,9 +1421,166 @@ template <int CG, int DG> struct Kernel {
         }
     }
 
+    // BBMStressUpdateStep::stressUpdateHighOrder (BBMStressUpdateStep.hpp:33-197): at the
+    // Gauss points, the elastic increment of the stress, the viscous relaxation factor, the
+    // Mohr-Coulomb and compressive failure criterion, the damage and stress relaxation; then
+    // the stress projected with iMJwPSI and the damage with iMJwPSI_dam. deltaT is the
+    // substep.
+    void bbmStressUpdate()
+    {
+        const MEBParams& P = meb;
+        const auto& PA = PsiT<DG, NG>::get();
+        const auto& PS = PsiT<S, NG>::get();
+        const double dt = deltaT, nu = P.nu0;
+#pragma omp parallel for
+        for (size_t i = 0; i < mesh.nelements; ++i) {
+            double hG[Q], aG[Q], dG[Q], e11G[Q], e12G[Q], e22G[Q], s11G[Q], s12G[Q], s22G[Q];
+            for (int q = 0; q < Q; ++q) {
+                double h = 0, a = 0, d = 0, x11 = 0, x12 = 0, x22 = 0, y11 = 0, y12 = 0, y22 = 0;
+                for (int k = 0; k < DG; ++k) {
+                    h += hice[i * DG + k] * PA.v[k][q];
+                    a += cice[i * DG + k] * PA.v[k][q];
+                    d += damage[i * DG + k] * PA.v[k][q];
+                }
+                for (int k = 0; k < S; ++k) {
+                    x11 += e11[i * S + k] * PS.v[k][q];
+                    x12 += e12[i * S + k] * PS.v[k][q];
+                    x22 += e22[i * S + k] * PS.v[k][q];
+                    y11 += s11[i * S + k] * PS.v[k][q];
+                    y12 += s12[i * S + k] * PS.v[k][q];
+                    y22 += s22[i * S + k] * PS.v[k][q];
+                }
+                hG[q] = std::max(h, 0.0);
+                aG[q] = std::min(std::max(a, 0.0), 1.0);
+                dG[q] = std::min(std::max(d, 1e-12), 1.0);
+                e11G[q] = x11;
+                e12G[q] = x12;
+                e22G[q] = x22;
+                s11G[q] = y11;
+                s12G[q] = y12;
+                s22G[q] = y22;
+            }
+            const double hE = hElem[i];
+            const double scale = std::sqrt(0.1 / hE);
+            for (int q = 0; q < Q; ++q) {
+                const double sn0 = 0.5 * (s11G[q] + s22G[q]);
+                const double expC = std::exp(P.compaction_param * (1.0 - aG[q]));
+                const double tv = P.undamaged_time_relaxation_sigma
+                    * std::pow(dG[q] * expC, P.exponent_relaxation_sigma - 1);
+                const double Pmax = P.P0 * std::pow(hG[q], P.exponent_compression_factor + 1.) * expC;
+                const double tildeP = sn0 < 0.0 ? std::min(-Pmax / sn0, 1.0) : 0.0;
+                const double mult = tv / (tv + (1. - tildeP) * dt);
+                const double elasticity = hG[q] * P.young * dG[q] * expC;
+                const double Du = dt * elasticity / (1. - nu * nu);
+                s11G[q] += Du * (e11G[q] + nu * e22G[q]);
+                s22G[q] += Du * (nu * e11G[q] + e22G[q]);
+                s12G[q] += Du * e12G[q] * (1. - nu);
+                s11G[q] *= mult;
+                s22G[q] *= mult;
+                s12G[q] *= mult;
+                const double sn = 0.5 * (s11G[q] + s22G[q]);
+                const double tau = std::sqrt(0.25 * (s11G[q] - s22G[q]) * (s11G[q] - s22G[q]) + s12G[q] * s12G[q]);
+                const double cohesion = P.C_lab * scale * hG[q];
+                const double cstr = P.compr_strength * scale * hG[q];
+                double dcrit = (tau + P.tan_phi * sn > 0.) ? cohesion / (tau + P.tan_phi * sn) : 1.;
+                dcrit = (sn < -cstr) ? -cstr / sn : dcrit;
+                dcrit = std::min(dcrit, 1.0);
+                const double td = hE * std::sqrt(2. * (1. + nu) * params.rho_ice) / std::sqrt(elasticity);
+                dG[q] -= dG[q] * (1. - dcrit) * dt / td;
+                s11G[q] -= s11G[q] * (1. - dcrit) * dt / td;
+                s12G[q] -= s12G[q] * (1. - dcrit) * dt / td;
+                s22G[q] -= s22G[q] * (1. - dcrit) * dt / td;
+            }
+            const auto& B = pmap->iMJwPSI[i];
+            for (int k = 0; k < S; ++k) {
+                double a = 0, b = 0, c = 0;
+                for (int q = 0; q < Q; ++q) {
+                    a += B[k * Q + q] * s11G[q];
+                    b += B[k * Q + q] * s12G[q];
+                    c += B[k * Q + q] * s22G[q];
+                }
+                s11[i * S + k] = a;
+                s12[i * S + k] = b;
+                s22[i * S + k] = c;
+            }
+            for (int k = 0; k < DG; ++k) {
+                double a = 0;
+                for (int q = 0; q < Q; ++q)
+                    a += iMJwPSIdam[i][k * Q + q] * dG[q];
+                damage[i * DG + k] = a;
+            }
+        }
+    }
+
+    // BrittleCGDynamicsKernel::updateMomentum (BrittleCGDynamicsKernel.hpp:200-253): the
+    // substep's semi-implicit update with the Coriolis and ocean-turning rotation, written
+    // as the reference has it (including the sign of the beta (gradX + tauX) term of v),
+    // and the accumulation of the step-averaged velocity.
+    void brittleMomentum()
+    {
+        const Params& p = params;
+        const double th = p.ocean_turning_angle * 0x1.1df46a2529d39p-6;
+        const double ct = std::cos(th), st = std::sin(th);
+        const auto& ML = pmap->lumped;
+        const double ns = static_cast<double>(p.nSteps);
+#pragma omp parallel for
+        for (size_t i = 0; i < u.size(); ++i) {
+            const double dtm = deltaT / (p.rho_ice * cgH[i]);
+            const double uI = u[i], vI = v[i];
+            const double cP = cgA[i] * p.F_ocean * std::hypot(uOcean[i] - uI, vOcean[i] - vI);
+            const double al = 1 + dtm * (cP * ct + 0.0);
+            const double be = deltaT * p.fc + dtm * cP * st;
+            const double rD = 1 / (al * al + be * be);
+            const double dA = cgA[i] * p.F_atm * std::hypot(uAtmos[i], vAtmos[i]);
+            const double tX = dA * uAtmos[i] + cP * (uOcean[i] * ct - vOcean[i] * st);
+            const double tY = dA * vAtmos[i] + cP * (vOcean[i] * ct + uOcean[i] * st);
+            const double gX = dSx[i] / ML[i] - p.rho_ice * cgH[i] * p.gravity * gradX[i];
+            const double gY = dSy[i] / ML[i] - p.rho_ice * cgH[i] * p.gravity * gradY[i];
+            double un = al * uI + be * vI + dtm * (al * (gX + tX) + be * (gY + tY));
+            un *= rD;
+            double vn = al * vI - be * uI + dtm * (al * (gY + tY) + be * (gX + tX));
+            vn *= rD;
+            u[i] = un;
+            v[i] = vn;
+            avgU[i] += un / ns;
+            avgV[i] += vn / ns;
+        }
+    }
+
+    // BrittleCGDynamicsKernel::update (BrittleCGDynamicsKernel.hpp:92-134)
+    void updateBBM(double dt)
+    {
+        const size_t ne = mesh.nelements;
+        advectionAndLimits(dt);
+        CG2DG<CG, S>(mesh, stressTr->velx.data(), avgU.data());
+        CG2DG<CG, S>(mesh, stressTr->vely.data(), avgV.data());
+        stressTr->reinitnormalvelocity();
+        stressTr->step(dt, s11.data());
+        stressTr->step(dt, s12.data());
+        stressTr->step(dt, s22.data());
+        tr->step(dt, damage.data());
+        limitMax<DG>(damage.data(), ne, 1.0);
+        limitMin<DG>(damage.data(), ne, 1e-12);
+        prepareIteration();
+        deltaT = dt / params.nSteps;
+        std::fill(avgU.begin(), avgU.end(), 0.0);
+        std::fill(avgV.begin(), avgV.end(), 0.0);
+        for (int k = 0; k < params.nSteps; ++k) {
+            projectVelocityToStrain();
+            bbmStressUpdate();
+            stressDivergence();
+            brittleMomentum();
+            applyBoundaries();
+        }
+    }
+
     // MEVPStressUpdateStep::stressUpdateHighOrder (MEVPStressUpdateStep.hpp:30-118)
     void stressUpdate()
     {
+        if (rheology == 1) {
+            bbmStressUpdate();
+            return;
+        }
         const auto& PA = PsiT<DG, NG>::get();
         const auto& PS = PsiT<S, NG>::get();
         const double alpha = params.alpha;
@@ -1460,6 +1699,10 @@ template <int CG, int DG> struct Kernel {
     // same-component Coriolis terms (-rho h fc u for u, +rho h fc v for v).
     void updateMomentum()
     {
+        if (rheology == 1) {
+            brittleMomentum();
+            return;
+        }
         const Params& p = params;
         const double beta = p.beta, SC = 1.0;
         const auto& ML = pmap->lumped;
@@ -1498,6 +1741,10 @@ template <int CG, int DG> struct Kernel {
     // VPCGDynamicsKernel::update (VPCGDynamicsKernel.hpp:63-94)
     void update(double dt)
     {
+        if (rheology == 1) {
+            updateBBM(dt);
+            return;
+        }
         advectionAndLimits(dt);
         prepareIteration();
         u0 = u;

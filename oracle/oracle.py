@@ -68,6 +68,7 @@ def load():
         "ora_advection_test": (ctypes.c_double, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
         "ora_advection_ring_test": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, _dp]),
         "ora_dyn_new": (_vp, [ctypes.c_int, ctypes.c_int, _dp]),
+        "ora_dyn_new_bbm": (_vp, [ctypes.c_int, ctypes.c_int, _dp, _dp]),
         "ora_dyn_free": (None, [_vp]),
         "ora_dyn_initialise": (ctypes.c_int, [_vp, _sz, _sz, _dp, _dp]),
         "ora_dyn_initialise_mesh": (ctypes.c_int, [_vp, _vp]),
@@ -224,12 +225,20 @@ class OracleTransport:
         return x.reshape(-1, e), y.reshape(-1, e)
 
 
+MEB_ORDER = ("compaction_param", "nu0", "young", "P0", "undamaged_time_relaxation_sigma",
+             "exponent_relaxation_sigma", "exponent_compression_factor", "tan_phi", "compr_strength", "C_lab")
+MEB_DEFAULTS = dict(compaction_param=-20.0, nu0=1.0 / 3.0, young=5.96e8, P0=10.e3,
+                    undamaged_time_relaxation_sigma=1e7, exponent_relaxation_sigma=5.0,
+                    exponent_compression_factor=1.5, tan_phi=0.7, compr_strength=1e10, C_lab=2.0e6)
+
+
 class OracleDyn:
-    """MEVPDynamicsKernel restatement (VPCGDynamicsKernel.hpp / CGDynamicsKernel.cpp)."""
+    """MEVPDynamicsKernel restatement (VPCGDynamicsKernel.hpp / CGDynamicsKernel.cpp);
+    with bbm=True the BBMDynamicsKernel (BrittleCGDynamicsKernel.hpp, BBMStressUpdateStep.hpp)."""
 
     PASSES = {"strain": 0, "stress": 1, "divergence": 2, "momentum": 3, "boundaries": 4, "subcycle": 5}
 
-    def __init__(self, cg, dg, params=None):
+    def __init__(self, cg, dg, params=None, bbm=False, meb=None):
         self.lib = load()
         self.cg, self.dg = cg, dg
         self.S = 3 if cg == 1 else 8
@@ -237,7 +246,12 @@ class OracleDyn:
         if params is not None:
             arr = np.array([float(params[k]) for k in PARAM_ORDER])
             p = arr
-        self.h = self.lib.ora_dyn_new(cg, dg, _p(p) if p is not None else None)
+        if bbm:
+            m = np.array([float(dict(MEB_DEFAULTS, **(meb or {}))[k]) for k in MEB_ORDER])
+            self._meb = m
+            self.h = self.lib.ora_dyn_new_bbm(cg, dg, _p(p) if p is not None else None, _p(m))
+        else:
+            self.h = self.lib.ora_dyn_new(cg, dg, _p(p) if p is not None else None)
         if not self.h:
             raise RuntimeError("oracle: " + self.lib.ora_last_error().decode())
         self.nx = self.ny = None
