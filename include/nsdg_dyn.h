@@ -116,6 +116,21 @@ int nsdg_limit(int dg, size_t n, double* phi, double bound, int is_max);
  * reference's CMake produces: (cg_degree 1, dg_adv 3, 3 stress dofs) and
  * (cg_degree 2, dg_adv 6, 8 stress dofs). params may be NULL (defaults). */
 int nsdg_dyn_create(int cg_degree, int dg_adv, const nsdg_params* params, nsdg_dyn** out);
+/* Brittle rheology parameters, MEBParameters (MEBParameters.hpp:15-70). */
+typedef struct nsdg_meb_params {
+    double compaction_param, nu0, young, P0, undamaged_time_relaxation_sigma;
+    double exponent_relaxation_sigma, exponent_compression_factor, tan_phi, compr_strength, C_lab;
+} nsdg_meb_params;
+int nsdg_meb_params_default(nsdg_meb_params* p);
+/* BBMDynamicsKernel<DG> (BBMDynamicsKernel.hpp:18-45 over BrittleCGDynamicsKernel.hpp:23-256),
+ * behind the same handle and calls as nsdg_dyn_create's kernel. update(dt) advects h and A
+ * with the velocity averaged over the previous step's substeps, transports the stress
+ * (DG 8, RK2) and the damage (then limits it to [1e-12, 1]), and runs params->nsteps
+ * brittle substeps of dt / nsteps (strain, BBM stress and damage update, divergence,
+ * semi-implicit momentum, Dirichlet zero). Extra fields: "damage" (DG, set_field /
+ * get_field; starts undamaged, 1), "avgU", "avgV" (get_cg). The ice-ocean stress comes from
+ * the averaged velocity. Uniform meshes only (NSDG_ERR_UNSUPPORTED otherwise). */
+int nsdg_bbm_create(int cg_degree, int dg_adv, const nsdg_params* params, const nsdg_meb_params* meb, nsdg_dyn** out);
 /* initialise(coords, isSpherical, mask) (DynamicsKernel.hpp:44-75,
  * CGDynamicsKernel.cpp:23-51). Builds the mesh, its Dirichlet lists, the transport and
  * momentum operators on the device. Stress and strain start at zero. */

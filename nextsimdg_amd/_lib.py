@@ -28,6 +28,16 @@ class NsdgParams(ctypes.Structure):
         "fc", "ocean_turning_angle", "gravity", "Pstar", "DeltaMin", "alpha", "beta")] + [("nsteps", ctypes.c_int32)]
 
 
+MEB_FIELDS = ("compaction_param", "nu0", "young", "P0", "undamaged_time_relaxation_sigma",
+              "exponent_relaxation_sigma", "exponent_compression_factor", "tan_phi", "compr_strength", "C_lab")
+
+
+class NsdgMEBParams(ctypes.Structure):
+    """nsdg_meb_params (include/nsdg_dyn.h)."""
+
+    _fields_ = [(n, ctypes.c_double) for n in MEB_FIELDS]
+
+
 # name -> (restype, argtypes); the full C ABI of include/nsdg_dyn.h
 SIGNATURES = {
     "nsdg_last_error": (_s, []),
@@ -51,6 +61,8 @@ SIGNATURES = {
     "nsdg_transport_destroy": (_i, [_vp]),
     "nsdg_limit": (_i, [_i, _sz, _dp, _d, _i]),
     "nsdg_dyn_create": (_i, [_i, _i, ctypes.POINTER(NsdgParams), ctypes.POINTER(_vp)]),
+    "nsdg_meb_params_default": (_i, [ctypes.POINTER(NsdgMEBParams)]),
+    "nsdg_bbm_create": (_i, [_i, _i, ctypes.POINTER(NsdgParams), ctypes.POINTER(NsdgMEBParams), ctypes.POINTER(_vp)]),
     "nsdg_dyn_initialise": (_i, [_vp, _sz, _sz, _dp, _i, _dp]),
     "nsdg_dyn_initialise_mesh": (_i, [_vp, _vp]),
     "nsdg_dyn_set_field": (_i, [_vp, _s, _dp, _i]),

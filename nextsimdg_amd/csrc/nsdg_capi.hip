@@ -284,6 +284,46 @@ int nsdg_dyn_create(int cg_degree, int dg_adv, const nsdg_params* params, nsdg_d
         *out = d.release();
     });
 }
+int nsdg_meb_params_default(nsdg_meb_params* p)
+{
+    return guard([&] {
+        nonNull(p, "params");
+        p->compaction_param = -20.0;
+        p->nu0 = 1.0 / 3.0;
+        p->young = 5.96e8;
+        p->P0 = 10.e3;
+        p->undamaged_time_relaxation_sigma = 1e7;
+        p->exponent_relaxation_sigma = 5.0;
+        p->exponent_compression_factor = 1.5;
+        p->tan_phi = 0.7;
+        p->compr_strength = 1e10;
+        p->C_lab = 2.0e6;
+    });
+}
+int nsdg_bbm_create(int cg_degree, int dg_adv, const nsdg_params* params, const nsdg_meb_params* meb, nsdg_dyn** out)
+{
+    return guard([&] {
+        nonNull(out, "out");
+        needDevice();
+        nsdg_params p;
+        if (params)
+            p = *params;
+        else
+            nsdg_params_default(&p);
+        require(p.nsteps >= 1, "nsteps must be >= 1");
+        nsdg_meb_params m;
+        if (meb)
+            m = *meb;
+        else
+            nsdg_meb_params_default(&m);
+        auto d = std::make_unique<nsdg_dyn>();
+        d->k = makeDyn(cg_degree, dg_adv, p);
+        d->k->setBrittle(m);
+        d->cg = cg_degree;
+        d->dg = dg_adv;
+        *out = d.release();
+    });
+}
 int nsdg_dyn_initialise(nsdg_dyn* d, size_t nx, size_t ny, const double* coords, int spherical, const double* mask)
 {
     return guard([&] {

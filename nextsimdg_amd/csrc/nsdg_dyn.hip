@@ -1526,6 +1526,163 @@ __global__ void k_mass_final(const double* part, int nb, double* out)
 }
 
 // ---------------------------------------------------------------- host object
+// ---------------------------------------------------------------- brittle (BBM) rheology
+// Constants of BBMStressUpdateStep::stressUpdateHighOrder (BBMStressUpdateStep.hpp:40-140)
+// for one substep on a uniform mesh: the element size h_e enters through sqrt(0.1 / h_e)
+// (cohesion and compressive strength) and h_e sqrt(2 (1 + nu) rho_ice) (damage time).
+struct BrittleArgs {
+    double dt, nu, young, P0, expP, compaction, tau0, expR, tanPhi, cohesion, compr, tdNum;
+    const double* damageIn;
+    double* damage; // SoA [DG][ne]
+};
+// One element of the BBM stress and damage update. Strain coefficients from the velocity
+// (STRAIN, ocean elements; stored) or the stored ones (land; a stress pass alone); at each Gauss point of the stress
+// rule the elastic increment, the viscous relaxation (tildeP from the compressive limit
+// Pmax), the Mohr-Coulomb / compressive failure d_crit and the damage relaxation over the
+// time t_d; then the stress projected with M^-1 PSI diag(w J) from the operator table and
+// the damage with psi_d w / m_d (exact for the affine elements of a uniform mesh).
+template <int CG, int DG, bool STRAIN>
+__global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs b)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, NG = L::G, Q = L::Q;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne)
+        return;
+    const size_t ne = a.ne;
+    const Ops<true> op { a.ops, a.stride };
+    double e11[S], e12[S], e22[S];
+    if (STRAIN && a.ocean[e]) {
+        strainLocal<CG, true>(a, e, e11, e12, e22);
+        storeTensor<S>(a.epsOut, ne, e, e11, e12, e22);
+    } else {
+        loadTensor<S>(a.eps, ne, e, e11, e12, e22);
+    }
+    double s11[S], s12[S], s22[S], hc[DG], ac[DG], dc[DG];
+    loadTensor<S>(a.sig, ne, e, s11, s12, s22);
+#pragma unroll
+    for (int i = 0; i < DG; ++i) {
+        hc[i] = a.h[i * ne + e];
+        ac[i] = a.A[i * ne + e];
+        dc[i] = b.damageIn[i * ne + e];
+    }
+    double acc1[S], acc2[S], acc3[S], accD[DG];
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        acc1[k] = acc2[k] = acc3[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < DG; ++k)
+        accD[k] = 0.0;
+    const double nu = b.nu, dt = b.dt;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double hg = 0, ag = 0, dg = 0;
+#pragma unroll
+        for (int i = 0; i < DG; ++i) {
+            hg += hc[i] * Psi<DG, NG>::v[i][q];
+            ag += ac[i] * Psi<DG, NG>::v[i][q];
+            dg += dc[i] * Psi<DG, NG>::v[i][q];
+        }
+        double x11 = 0, x12 = 0, x22 = 0, y11 = 0, y12 = 0, y22 = 0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double ps = Psi<S, NG>::v[k][q];
+            x11 += e11[k] * ps;
+            x12 += e12[k] * ps;
+            x22 += e22[k] * ps;
+            y11 += s11[k] * ps;
+            y12 += s12[k] * ps;
+            y22 += s22[k] * ps;
+        }
+        hg = fmax(hg, 0.0);
+        ag = fmin(fmax(ag, 0.0), 1.0);
+        dg = fmin(fmax(dg, 1e-12), 1.0);
+        const double sn0 = 0.5 * (y11 + y22);
+        const double expC = exp(b.compaction * (1.0 - ag));
+        const double tv = b.tau0 * pow(dg * expC, b.expR);
+        const double Pmax = b.P0 * pow(hg, b.expP) * expC;
+        const double tildeP = sn0 < 0.0 ? fmin(-Pmax / sn0, 1.0) : 0.0;
+        const double mult = tv / (tv + (1.0 - tildeP) * dt);
+        const double E = hg * b.young * dg * expC;
+        const double Du = dt * E / (1.0 - nu * nu);
+        y11 = (y11 + Du * (x11 + nu * x22)) * mult;
+        y22 = (y22 + Du * (nu * x11 + x22)) * mult;
+        y12 = (y12 + Du * x12 * (1.0 - nu)) * mult;
+        const double sn = 0.5 * (y11 + y22);
+        const double tau = sqrt(0.25 * (y11 - y22) * (y11 - y22) + y12 * y12);
+        const double cohesion = b.cohesion * hg, cstr = b.compr * hg;
+        double dcrit = (tau + b.tanPhi * sn > 0.0) ? cohesion / (tau + b.tanPhi * sn) : 1.0;
+        dcrit = sn < -cstr ? -cstr / sn : dcrit;
+        dcrit = fmin(dcrit, 1.0);
+        const double td = b.tdNum / sqrt(E);
+        const double relax = (1.0 - dcrit) * dt / td;
+        dg -= dg * relax;
+        y11 -= y11 * relax;
+        y12 -= y12 * relax;
+        y22 -= y22 * relax;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double w = op(L::B + k * Q + q, e);
+            acc1[k] += w * y11;
+            acc2[k] += w * y12;
+            acc3[k] += w * y22;
+        }
+#pragma unroll
+        for (int k = 0; k < DG; ++k)
+            accD[k] += (Psi<DG, NG>::v[k][q] * (gauss_weight(NG, q % NG) * gauss_weight(NG, q / NG))
+                           * invMassRef(k))
+                * dg;
+    }
+    storeTensor<S>(a.sig, ne, e, acc1, acc2, acc3);
+#pragma unroll
+    for (int k = 0; k < DG; ++k)
+        b.damage[k * ne + e] = accD[k];
+}
+
+// BrittleCGDynamicsKernel::updateMomentum (BrittleCGDynamicsKernel.hpp:200-253) for one
+// node: the semi-implicit substep with the Coriolis term and the ocean drag turned by the
+// turning angle, solved as the reference writes it (its v carries +beta (gradX + tauX)),
+// and the step-averaged velocity accumulated before the Dirichlet zero.
+__global__ __launch_bounds__(kBlock) void k_brittle_momentum(NodeArgs a, double ct, double st, double nsteps,
+    double* avgU, double* avgV)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nn)
+        return;
+    const double H = a.H[n], Ac = a.A[n], uo = a.uo[n], vo = a.vo[n], ua = a.ua[n], va = a.va[n];
+    const double uI = a.u[n], vI = a.v[n];
+    const double dtm = a.dt / (a.rho_ice * H);
+    const double cP = Ac * a.F_ocean * hypot(uo - uI, vo - vI);
+    const double al = 1.0 + dtm * (cP * ct);
+    const double be = a.dt * a.fc + dtm * cP * st;
+    const double rD = 1.0 / (al * al + be * be);
+    const double dA = Ac * a.F_atm * hypot(ua, va);
+    const double tX = dA * ua + cP * (uo * ct - vo * st);
+    const double tY = dA * va + cP * (vo * ct + uo * st);
+    const double gX = a.dSx[n] / a.ML[n] - a.rho_ice * H * a.gravity * a.gx[n];
+    const double gY = a.dSy[n] / a.ML[n] - a.rho_ice * H * a.gravity * a.gy[n];
+    const double un = (al * uI + be * vI + dtm * (al * (gX + tX) + be * (gY + tY))) * rD;
+    const double vn = (al * vI - be * uI + dtm * (al * (gY + tY) + be * (gX + tX))) * rD;
+    a.u[n] = un;
+    a.v[n] = vn;
+    avgU[n] += un / nsteps;
+    avgV[n] += vn / nsteps;
+}
+
+// BrittleCGDynamicsKernel::getIceOceanStress (BrittleCGDynamicsKernel.hpp:163-182): from
+// the step-averaged velocity, relative velocity ocean minus ice, turned.
+__global__ void k_ice_ocean_stress_brittle(int nn, double F_ocean, double ct, double st, const double* au,
+    const double* av, const double* uo, const double* vo, double* tx, double* ty)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn)
+        return;
+    const double ur = uo[n] - au[n], vr = vo[n] - av[n];
+    const double cp = F_ocean * hypot(ur, vr);
+    tx[n] = cp * (ur * ct - vr * st);
+    ty[n] = cp * (vr * ct + ur * st);
+}
+
 template <int CG, int DG> struct DynImpl final : DynBase {
     using L = MomLayout<CG>;
     static constexpr int S = L::S, N = L::N;
@@ -1552,6 +1709,14 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<double> uStart, vStart, sigStart; // saved start of a timestep (warm-start gate)
     DBuf<double> uAlt, vAlt, sigAlt;       // ping-pong partners of u, v, sigma (fused subcycle)
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
+    // brittle rheology (BBMDynamicsKernel): damage (SoA DG), step-averaged velocity,
+    // stress transport in the stress space, element size of the uniform mesh
+    bool brittle = false;
+    nsdg_meb_params meb {};
+    DBuf<double> damage, avgU, avgV;
+    std::unique_ptr<TransportBase> stressTr;
+    CG2DGOps cg2dgS;
+    double hElem = 0.0;
 
     explicit DynImpl(const nsdg_params& prm)
         : p(prm)
@@ -1561,6 +1726,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     ~DynImpl() override
     {
         tr.reset();
+        stressTr.reset();
         if (s)
             (void)hipStreamDestroy(s);
     }
@@ -1639,6 +1805,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             }
         dirNode.alloc(nn);
         dirNode.upload(dn.data(), nn, s);
+        if (brittle)
+            initBrittle();
         passive.clear();
         subcycleCount = 0;
         NSDG_HIP(hipStreamSynchronize(s));
@@ -1654,6 +1822,48 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             allocPrecision();
         }
     }
+
+    // BrittleCGDynamicsKernel::initialise (BrittleCGDynamicsKernel.hpp:72-87): undamaged
+    // ice (damage 1), zero averaged velocity, the stress transport on the stress space.
+    void initBrittle()
+    {
+        require(mesh.uniform, "the brittle rheology needs a uniform (affine) mesh", NSDG_ERR_UNSUPPORTED);
+        stressTr = makeTransport(S, mesh, s, prof);
+        stressTr->setScheme(2);
+        cg2dgS.build(CG, S, mesh, s);
+        damage.alloc(ne * DG);
+        deviceFill(damage.p, ne * DG, 0.0, s);
+        deviceFill(damage.p, ne, 1.0, s);
+        avgU.alloc(nn);
+        avgV.alloc(nn);
+        deviceFill(avgU.p, nn, 0.0, s);
+        deviceFill(avgV.p, nn, 0.0, s);
+        // sqrt of the quadrilateral area (ParametricMesh.hpp:276-299) of element 0, shared
+        // by every translate of it
+        const double* c = mesh.host.coords.data();
+        const size_t vx = mesh.host.nx + 1;
+        const size_t id[4] = { 0, 1, vx, vx + 1 }; // lower left, lower right, upper left, upper right
+        auto sq = [&](int i, int j) {
+            const double dx = c[2 * id[i]] - c[2 * id[j]], dy = c[2 * id[i] + 1] - c[2 * id[j] + 1];
+            return dx * dx + dy * dy;
+        };
+        const double a = sq(0, 1), b = sq(1, 3), cc = sq(2, 3), d = sq(0, 2), e = sq(0, 3), f = sq(1, 2);
+        hElem = std::sqrt(0.25 * std::sqrt(4.0 * e * f - (b + d - a - cc) * (b + d - a - cc)));
+    }
+    void setBrittle(const nsdg_meb_params& m) override
+    {
+        require(!ready, "the rheology is chosen before initialise", NSDG_ERR_STATE);
+        require(m.young > 0 && m.nu0 >= 0 && m.nu0 < 1 && m.undamaged_time_relaxation_sigma > 0,
+            "MEB parameters need young > 0, 0 <= nu0 < 1 and a positive relaxation time");
+        brittle = true;
+        meb = m;
+    }
+    void needMEVP(const char* what) const
+    {
+        require(!brittle, std::string(what) + " is an mEVP facility", NSDG_ERR_UNSUPPORTED);
+    }
+    double turnCos() const { return std::cos(p.ocean_turning_angle * 0x1.1df46a2529d39p-6); }
+    double turnSin() const { return std::sin(p.ocean_turning_angle * 0x1.1df46a2529d39p-6); }
 
     // ---- field access
     double* cgRef(const std::string& n)
@@ -1673,6 +1883,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         if (n == "uwind") return ua.p;
         if (n == "vwind") return va.p;
         if (n == "lumpedcgmass") return lumped.p;
+        if (brittle && n == "avgU") return avgU.p;
+        if (brittle && n == "avgV") return avgV.p;
         return nullptr;
     }
     double* tensorRef(const std::string& n)
@@ -1711,9 +1923,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
                 ? cgRef(n)
                 : nullptr) {
             dg0ToCG(data, ncomp, c);
-        } else if (n == "hice" || n == "cice") {
-            double* f = n == "hice" ? h.p : A.p;
-            require(ncomp == 1 || ncomp == DG, "hice/cice need 1 or DG components");
+        } else if (n == "hice" || n == "cice" || (brittle && n == "damage")) {
+            double* f = n == "hice" ? h.p : (n == "cice" ? A.p : damage.p);
+            require(ncomp == 1 || ncomp == DG, "hice/cice/damage need 1 or DG components");
             if (ncomp == DG)
                 uploadSoA(scratch, data, ne, DG, f, s);
             else {
@@ -1744,7 +1956,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         if (n == "u" || n == "v" || n == "uiostress" || n == "viostress") {
             require(ncomp == 1 || ncomp == DG, "ncomp must be 1 or DG");
             const double* src = n == "u" ? u.p : v.p;
-            if (n == "uiostress" || n == "viostress") {
+            if ((n == "uiostress" || n == "viostress") && brittle) {
+                k_ice_ocean_stress_brittle<<<blocksFor(nn), kBlock, 0, s>>>(static_cast<int>(nn), p.F_ocean,
+                    turnCos(), turnSin(), avgU.p, avgV.p, uo.p, vo.p, tmpCG.p, tmpCG2.p);
+                checkLaunch("ice_ocean_stress");
+                src = n == "uiostress" ? tmpCG.p : tmpCG2.p;
+            } else if (n == "uiostress" || n == "viostress") {
                 k_ice_ocean_stress<<<blocksFor(nn), kBlock, 0, s>>>(
                     static_cast<int>(nn), p.F_ocean, u.p, v.p, uo.p, vo.p, tmpCG.p, tmpCG2.p);
                 checkLaunch("ice_ocean_stress");
@@ -1752,9 +1969,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             }
             cg2dg.apply(mesh, src, tmpDG.p, nullptr, nullptr, s, prof);
             downloadSoA(scratch, tmpDG.p, ne, ncomp == 1 ? 1 : DG, out, s);
-        } else if (n == "hice" || n == "cice") {
+        } else if (n == "hice" || n == "cice" || (brittle && n == "damage")) {
             require(ncomp == 1 || ncomp == DG, "ncomp must be 1 or DG");
-            downloadSoA(scratch, n == "hice" ? h.p : A.p, ne, ncomp, out, s);
+            downloadSoA(scratch, n == "hice" ? h.p : (n == "cice" ? A.p : damage.p), ne, ncomp, out, s);
         } else if (n == "ssh") {
             require(ncomp == 1, "ssh is a DG0 field");
             downloadSoA(scratch, ssh.p, ne, 1, out, s);
@@ -1793,9 +2010,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     }
 
     // ---- the time step
+    // DynamicsKernel::advectionAndLimits (DynamicsKernel.hpp:160-172); the brittle kernel
+    // advects with the velocity averaged over the previous step's substeps
+    // (BrittleCGDynamicsKernel.hpp:89-90).
     void advectAsync(double dt)
     {
-        cg2dg.apply(mesh, u.p, tr->velx(), v.p, tr->vely(), s, prof);
+        cg2dg.apply(mesh, brittle ? avgU.p : u.p, tr->velx(), brittle ? avgV.p : v.p, tr->vely(), s, prof);
         tr->reinitNormalVelocity();
         tr->step(dt, A.p);
         tr->step(dt, h.p);
@@ -2030,9 +2250,24 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         swapBuf(u, uAlt);
         swapBuf(v, vAlt);
     }
+    // The strain output of the reference is that of the velocity entering the last subcycle
+    // (projectVelocityToStrain runs first in every subcycle); the subcycle kernels keep it
+    // in registers, so it is stored once, before the last pass of a run.
+    void strainAsync()
+    {
+        Timed t(prof, s, "strain");
+        const ElemArgs a = elemArgs();
+        if (mesh.uniform)
+            k_strain<CG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        else
+            k_strain<CG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        checkLaunch("strain");
+    }
     void runPasses(int n)
     {
         for (int k = 0; k < n; ++k) {
+            if (k == n - 1)
+                strainAsync();
             if (fusedSubcycle()) {
                 fusedPass();
             } else {
@@ -2062,6 +2297,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     void subcyclesToResidual(double rTarget, int maxN, int every, int* done, double* rFinal) override
     {
         need();
+        needMEVP("the residual-driven subcycle loop");
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
         require(maxN >= 0 && every >= 1, "need max_subcycles >= 0 and check_every >= 1");
         double r = 0.0;
@@ -2084,10 +2320,76 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             *rFinal = r;
     }
 
+    // ---- brittle substeps (BrittleCGDynamicsKernel.hpp:92-134)
+    BrittleArgs brittleArgs() const
+    {
+        BrittleArgs b {};
+        b.dt = deltaT;
+        b.nu = meb.nu0;
+        b.young = meb.young;
+        b.P0 = meb.P0;
+        b.expP = meb.exponent_compression_factor + 1.0;
+        b.compaction = meb.compaction_param;
+        b.tau0 = meb.undamaged_time_relaxation_sigma;
+        b.expR = meb.exponent_relaxation_sigma - 1.0;
+        b.tanPhi = meb.tan_phi;
+        const double scale = std::sqrt(0.1 / hElem);
+        b.cohesion = meb.C_lab * scale;
+        b.compr = meb.compr_strength * scale;
+        b.tdNum = hElem * std::sqrt(2.0 * (1.0 + meb.nu0) * p.rho_ice);
+        b.damageIn = damage.p;
+        b.damage = damage.p;
+        return b;
+    }
+    template <bool STRAIN> void brittleStressAsync()
+    {
+        Timed t(prof, s, "bbm_element");
+        k_bbm_element<CG, DG, STRAIN><<<blocksFor(ne), kBlock, 0, s>>>(elemArgs(), brittleArgs());
+        checkLaunch("bbm_element");
+    }
+    void brittleMomentumAsync()
+    {
+        Timed t(prof, s, "brittle_momentum");
+        k_brittle_momentum<<<blocksFor(nn), kBlock, 0, s>>>(
+            nodeArgs(), turnCos(), turnSin(), static_cast<double>(p.nsteps), avgU.p, avgV.p);
+        checkLaunch("brittle_momentum");
+    }
+    void brittleSubstepsAsync(int n)
+    {
+        for (int k = 0; k < n; ++k) {
+            brittleStressAsync<true>();
+            divergenceAsync();
+            brittleMomentumAsync();
+            boundariesAsync();
+        }
+    }
+    // Advection of h and A with the averaged velocity, transport of the stress (stress
+    // space, RK2) and of the damage (then limited to [1e-12, 1]), then nsteps substeps of
+    // dt / nsteps from a zero average.
+    void updateBrittle(double dt)
+    {
+        advectAsync(dt);
+        cg2dgS.apply(mesh, avgU.p, stressTr->velx(), avgV.p, stressTr->vely(), s, prof);
+        stressTr->reinitNormalVelocity();
+        for (int c = 0; c < 3; ++c)
+            stressTr->step(dt, sig.p + static_cast<size_t>(c) * S * ne);
+        tr->step(dt, damage.p);
+        tr->limit(damage.p, true, 1.0, true, 1e-12);
+        prepareIterationAsync();
+        deltaT = dt / p.nsteps;
+        deviceFill(avgU.p, nn, 0.0, s);
+        deviceFill(avgV.p, nn, 0.0, s);
+        brittleSubstepsAsync(p.nsteps);
+    }
+
     void update(double dt) override
     {
         need();
         require(dt > 0, "dt must be positive");
+        if (brittle) {
+            updateBrittle(dt);
+            return;
+        }
         advectAsync(dt);
         prepareIterationAsync();
         beginIterationAsync(dt);
@@ -2126,6 +2428,13 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     void stress(int repeat) override
     {
         need();
+        if (brittle) {
+            require(deltaT > 0, "begin_iteration must set dt first", NSDG_ERR_STATE);
+            for (int r = 0; r < repeat; ++r)
+                brittleStressAsync<false>();
+            synchronize();
+            return;
+        }
         const ElemArgs a = elemArgs();
         for (int r = 0; r < repeat; ++r) {
             if (mesh.uniform)
@@ -2146,7 +2455,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     {
         need();
         require(deltaT > 0, "begin_iteration must set dt first", NSDG_ERR_STATE);
-        momentumAsync();
+        if (brittle)
+            brittleMomentumAsync();
+        else
+            momentumAsync();
         synchronize();
     }
     void boundaries() override
@@ -2159,7 +2471,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     {
         need();
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
-        subcyclesAsync(n);
+        if (brittle)
+            brittleSubstepsAsync(n);
+        else
+            subcyclesAsync(n);
     }
     // sigma_out <- Pi sigma_VP(eps(u)) (FP64, from the current h, A, u, v).
     void vpStressAsync(double* out)
@@ -2180,6 +2495,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     void residual(double* r, double* normR, double* normD) override
     {
         need();
+        needMEVP("the momentum residual");
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
         {
             Timed t(prof, s, "residual");
@@ -2221,6 +2537,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     void needCG2() const
     {
         need();
+        needMEVP("the learned warm start");
         require(CG == 2, "the learned warm start is built for CG2 velocities", NSDG_ERR_UNSUPPORTED);
     }
     void features(uint16_t* x) override
@@ -2274,6 +2591,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     void stressFromVelocity() override
     {
         need();
+        needMEVP("the viscous-plastic stress of a velocity");
         vpStressAsync(sig.p);
         synchronize();
     }
@@ -2281,6 +2599,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // uniform meshes with the fused node pass (no periodic segments).
     void checkPrecision() const
     {
+        if (precision == NSDG_PREC_FP32)
+            needMEVP("the FP32 variant");
         if (precision == NSDG_PREC_FP32)
             require(mesh.uniform && !mesh.hasPeriodic,
                 "the FP32 variant needs a uniform (affine) mesh without periodic segments", NSDG_ERR_UNSUPPORTED);
@@ -2371,8 +2691,14 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     };
     std::vector<std::pair<double*, size_t>> stateParts()
     {
-        return { { h.p, ne * DG }, { A.p, ne * DG }, { sig.p, 3 * S * ne }, { eps.p, 3 * S * ne }, { u.p, nn },
-            { v.p, nn }, { u0.p, nn }, { v0.p, nn } };
+        std::vector<std::pair<double*, size_t>> parts { { h.p, ne * DG }, { A.p, ne * DG }, { sig.p, 3 * S * ne },
+            { eps.p, 3 * S * ne }, { u.p, nn }, { v.p, nn }, { u0.p, nn }, { v0.p, nn } };
+        if (brittle) {
+            parts.push_back({ damage.p, ne * DG });
+            parts.push_back({ avgU.p, nn });
+            parts.push_back({ avgV.p, nn });
+        }
+        return parts;
     }
     std::vector<char> saveState() override
     {
@@ -2383,7 +2709,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         std::vector<char> buf(total);
         Header hd {};
         std::memcpy(hd.magic, "NSDGST01", 8);
-        hd.version = 1;
+        hd.version = brittle ? 2 : 1; // 2: brittle, with damage and averaged velocity
         hd.cg = CG;
         hd.dg = DG;
         hd.s = S;
@@ -2405,7 +2731,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         require(size >= sizeof(Header), "state blob too small");
         Header hd;
         std::memcpy(&hd, buf, sizeof hd);
-        require(std::memcmp(hd.magic, "NSDGST01", 8) == 0 && hd.version == 1, "not a state blob of this library");
+        require(std::memcmp(hd.magic, "NSDGST01", 8) == 0 && (hd.version == 1 || hd.version == 2),
+            "not a state blob of this library");
+        require(hd.version == (brittle ? 2 : 1), "state blob of the other rheology");
         require(hd.cg == CG && hd.dg == DG && hd.nx == static_cast<uint64_t>(mesh.nx)
                 && hd.ny == static_cast<uint64_t>(mesh.ny),
             "state blob does not match this kernel's degrees or mesh");

@@ -8,6 +8,8 @@ reference's own:
                         DynamicsKernel (DynamicsKernel.hpp:37-229), CGDynamicsKernel
                         (CGDynamicsKernel.hpp:26-86), VPCGDynamicsKernel
                         (VPCGDynamicsKernel.hpp:22-176)
+  BBMDynamicsKernel     dynamics/src/include/BBMDynamicsKernel.hpp:18-45 over
+                        BrittleCGDynamicsKernel (BrittleCGDynamicsKernel.hpp:23-256)
 Arrays are numpy float64 in the ModelArray layout (element ix + nx*iy, components
 contiguous). Errors raise NsdgError (the reference aborts or throws).
 """
@@ -16,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import NsdgParams, check, dptr, u64ptr
+from ._lib import MEB_FIELDS, NsdgMEBParams, NsdgParams, check, dptr, u64ptr
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -30,6 +32,17 @@ def default_params(**overrides):
     for k, v in overrides.items():
         if k == "nSteps":
             k = "nsteps"
+        setattr(p, k, v)
+    return p
+
+
+def default_meb_params(**overrides):
+    """MEBParameters defaults (MEBParameters.hpp:38-70)."""
+    p = NsdgMEBParams()
+    check(_lib.load().nsdg_meb_params_default(ctypes.byref(p)))
+    for k, v in overrides.items():
+        if k not in MEB_FIELDS:
+            raise KeyError(f"unknown MEB parameter {k}")
         setattr(p, k, v)
     return p
 
@@ -149,10 +162,13 @@ class MEVPDynamicsKernel:
         self.S = 3 if self.cg == 1 else 8
         p = params if params is not None else default_params(**overrides)
         h = ctypes.c_void_p()
-        check(self.lib.nsdg_dyn_create(self.cg, self.dg, ctypes.byref(p), ctypes.byref(h)))
+        self._create(p, h)
         self.h = h
         self.params = p
         self.nx = self.ny = None
+
+    def _create(self, p, h):
+        check(self.lib.nsdg_dyn_create(self.cg, self.dg, ctypes.byref(p), ctypes.byref(h)))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -284,3 +300,22 @@ class MEVPDynamicsKernel:
         c, t = ctypes.c_int(), ctypes.c_double()
         check(self.lib.nsdg_dyn_kernel_stats(self.h, name.encode(), ctypes.byref(c), ctypes.byref(t)))
         return c.value, t.value
+
+
+class BBMDynamicsKernel(MEVPDynamicsKernel):
+    """Brittle (BBM) dynamics kernel (BBMDynamicsKernel<DG>) with device-resident state.
+
+    update(dt) advects h and A with the velocity averaged over the previous step's
+    substeps, transports the stress and the damage, and runs nSteps brittle substeps of
+    dt / nSteps (BrittleCGDynamicsKernel.hpp:92-134). The extra DG field "damage" is set
+    and read like hice; the averaged velocity is the CG vector "avgU" / "avgV". The
+    mEVP-only facilities (momentum residual, FP32 variant, learned warm start) raise.
+    Uniform meshes only.
+    """
+
+    def __init__(self, cg_degree=2, dg_adv=6, params=None, meb=None, **overrides):
+        self.meb = meb if meb is not None else default_meb_params()
+        super().__init__(cg_degree, dg_adv, params, **overrides)
+
+    def _create(self, p, h):
+        check(self.lib.nsdg_bbm_create(self.cg, self.dg, ctypes.byref(p), ctypes.byref(self.meb), ctypes.byref(h)))

@@ -1,0 +1,220 @@
+"""GPU parity of the brittle (BBM) rheology: BBMDynamicsKernel on gfx950 (through the C ABI)
+against the oracle's BrittleCGDynamicsKernel restatement on the same seeded inputs.
+
+Tolerances: single passes within 1e-12 relative L2 per field; after 5 module-level steps
+of the benchmark box (each 100 substeps with stress and damage transport between steps)
+within 1e-10 per field (u, v, avgU, avgV, h, A, sigma, strain, damage, ice-ocean stress);
+after 10 steps, once failure has localised and the problem amplifies rounding, within 10x
+the spread of the oracle under a 1e-15 perturbation of its input.
+"""
+import numpy as np
+import pytest
+
+from helpers import DEFAULT_PARAMS, box_coords, rel_l2
+from oracle import MEB_DEFAULTS, Oracle, OracleDyn
+
+pytestmark = pytest.mark.gpu
+
+ORA = Oracle()
+CASES = [(2, 6), (1, 3)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _pair(cg, dg, nx, ny, coords, mask, params=None, meb=None):
+    import nextsimdg_amd as nd
+
+    p = dict(DEFAULT_PARAMS, **(params or {}))
+    m = dict(MEB_DEFAULTS, **(meb or {}))
+    o = OracleDyn(cg, dg, p, bbm=True, meb=m)
+    o.initialise(nx, ny, coords, mask)
+    g = nd.BBMDynamicsKernel(cg, dg, params=nd.default_params(**p), meb=nd.default_meb_params(**m))
+    g.initialise(coords, False, mask, nx, ny)
+    return o, g
+
+
+def _box_mask(nx, ny):
+    m = np.ones((ny, nx))
+    m[0, :] = m[-1, :] = m[:, 0] = m[:, -1] = 0
+    m[ny // 2, nx // 3] = 0  # an interior land cell
+    return m.ravel()
+
+
+def _set_state(o, g, rng, S, dg, uscale=0.1, sscale=3e3):
+    ne = o.ne
+    h = np.abs(rng.normal(1.0, 0.3, (ne, dg))) * np.r_[1.0, [0.05] * (dg - 1)]
+    A = np.clip(rng.normal(0.9, 0.05, (ne, dg)), 0, 1) * np.r_[1.0, [0.02] * (dg - 1)]
+    # damage below 0.9 with small slopes: its Gauss values stay inside (1e-12, 1)
+    d = np.clip(rng.normal(0.7, 0.1, (ne, dg)), 0.05, 0.9) * np.r_[1.0, [0.03] * (dg - 1)]
+    for name, v in (("hice", h), ("cice", A), ("damage", d)):
+        o.set_field(name, v, dg)
+        g.setData(name, v, dg)
+    for name in ("s11", "s12", "s22", "e11", "e12", "e22"):
+        v = rng.normal(0, sscale if name[0] == "s" else 1e-7, (ne, S))
+        o.set_field(name, v, S)
+        g.setData(name, v, S)
+    for name, scale in (("uwind", 10), ("vwind", 10), ("uocean", 0.05), ("vocean", 0.05)):
+        v = rng.normal(0, scale, ne)
+        o.set_field(name, v, 1)
+        g.setData(name, v, 1)
+    ssh = rng.normal(0, 0.1, ne)
+    o.set_field("ssh", ssh, 1)
+    g.setData("ssh", ssh, 1)
+    for name in ("u", "v"):
+        v = rng.normal(0, uscale, o.ncg)
+        o.set_cg(name, v)
+        g.set_cg(name, v)
+
+
+@pytest.mark.parametrize("cg,dg", CASES)
+@pytest.mark.parametrize("turning", [0.0, 25.0])
+def test_bbm_single_passes_match_oracle(cg, dg, turning):
+    _say(f"BBM single passes cg={cg} dg={dg} turning={turning}")
+    nx, ny = 21, 13
+    o, g = _pair(cg, dg, nx, ny, box_coords(nx, ny, 3000.0), _box_mask(nx, ny),
+                 params={"ocean_turning_angle": turning})
+    S = o.S
+    # stresses near the cohesion (11.5 kPa here): part of the elements fail, part do not
+    _set_state(o, g, np.random.default_rng(7 + cg), S, dg, uscale=1e-3, sscale=5e3)
+    o.prepare_iteration()
+    g.prepareIteration()
+    o.begin_iteration(2.0)
+    g.begin_iteration(2.0)
+    o.run_pass("strain")
+    g.projectVelocityToStrain()
+    for n in ("e11", "e12", "e22"):
+        assert rel_l2(g.getDGData(n, S), o.get_field(n, S)) < 1e-12, n
+    d0 = o.get_field("damage", dg).copy()
+    o.run_pass("stress", 2)
+    g.stressUpdateHighOrder(2)
+    d1 = o.get_field("damage", dg)
+    damaged = d1[:, 0] < d0[:, 0] - 1e-9
+    _say(f"  damaged elements: {damaged.sum()} of {o.ne}")
+    assert damaged.any() and np.isclose(d1[:, 0], d0[:, 0], rtol=1e-13).any()
+    for n in ("s11", "s12", "s22"):
+        assert rel_l2(g.getDGData(n, S), o.get_field(n, S)) < 1e-12, n
+    assert rel_l2(g.getDGData("damage", dg), d1) < 1e-12
+    o.run_pass("divergence")
+    g.stressDivergence()
+    for n in ("dStressX", "dStressY"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-12, n
+    o.run_pass("momentum")
+    g.updateMomentum()
+    for n in ("u", "v", "avgU", "avgV"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-12, n
+    o.run_pass("boundaries")
+    g.applyBoundaries()
+    for n in ("u", "v"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-12, n
+    for n in ("uiostress", "viostress"):
+        assert rel_l2(g.getDGData(n, dg), o.get_field(n, dg)) < 1e-12, n
+
+
+def _inputs(nx, ny, steps, dt, seed):
+    from nextsimdg_amd import inputs
+
+    setup = inputs.benchmark_setup(nx, ny, seed)
+    xoff, yoff = inputs.cyclone_offset(seed)
+    winds = [ORA.benchmark_wind(nx, ny, inputs.L_BOX, k * dt, xoff, yoff, 30.0) for k in range(steps)]
+    return setup, winds
+
+
+def _fields(k, dg, S, cg_get, dg_get):
+    out = {n: cg_get(k, n) for n in ("u", "v", "avgU", "avgV")}
+    out.update({n: dg_get(k, n, dg) for n in ("hice", "cice", "damage", "uiostress", "viostress")})
+    out.update({n: dg_get(k, n, S) for n in ("s11", "s12", "s22", "e11", "e12", "e22")})
+    return out
+
+
+def _oracle_run(cg, dg, nx, ny, setup, winds, dt, pert_seed=None, checkpoints=(5, 10)):
+    """The oracle over the benchmark box; pert_seed perturbs h by 1e-15 relative."""
+    from nextsimdg_amd import inputs
+
+    o = OracleDyn(cg, dg, DEFAULT_PARAMS, bbm=True, meb=MEB_DEFAULTS)
+    o.initialise(nx, ny, inputs.box_coords(nx, ny), inputs.box_mask(nx, ny))
+    for name, key in (("uocean", "uocean"), ("vocean", "vocean"), ("hice", "h"), ("cice", "A")):
+        v = setup[key].copy()
+        if pert_seed is not None and name == "hice":
+            v *= 1 + 1e-15 * np.random.default_rng(pert_seed).uniform(-1, 1, v.shape)
+        o.set_field(name, v, 1)
+    snaps = {}
+    for k, (uw, vw) in enumerate(winds):
+        for name, v in (("uwind", uw), ("vwind", vw), ("ssh", np.zeros(nx * ny))):
+            o.set_field(name, v, 1)
+        o.update(dt)
+        if k + 1 in checkpoints:
+            snaps[k + 1] = _fields(o, dg, o.S, lambda x, n: x.get_cg(n).copy(),
+                                   lambda x, n, c: x.get_field(n, c).copy())
+    return o, snaps
+
+
+@pytest.mark.parametrize("cg,dg", CASES)
+def test_bbm_model_steps_match_oracle(cg, dg):
+    """Steps of the benchmark box under the cyclone, BBM with its default parameters.
+
+    The first 5 steps: <= 1e-10 relative L2 per field. From about the 6th step the
+    brittle failure localises and the problem itself amplifies rounding differences by
+    ~1e5 per step (the oracle with h perturbed by 1e-15 departs from itself as far), so at
+    step 10 the device must lie within 10x the spread of two such perturbed oracle runs.
+    """
+    _say(f"BBM steps cg={cg} dg={dg}")
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    nx = ny = 24 if cg == 2 else 32
+    dt = 120.0
+    setup, winds = _inputs(nx, ny, 10, dt, seed=5)
+    o, ref = _oracle_run(cg, dg, nx, ny, setup, winds, dt)
+    spread = [_oracle_run(cg, dg, nx, ny, setup, winds, dt, pert_seed=s)[1] for s in (1, 2)]
+    g = nd.BBMDynamicsKernel(cg, dg, params=nd.default_params(**DEFAULT_PARAMS),
+                             meb=nd.default_meb_params(**MEB_DEFAULTS))
+    g.initialise(inputs.box_coords(nx, ny), False, inputs.box_mask(nx, ny), nx, ny)
+    for name, key in (("uocean", "uocean"), ("vocean", "vocean"), ("hice", "h"), ("cice", "A")):
+        g.setData(name, setup[key], 1)
+    S = o.S
+    for k, (uw, vw) in enumerate(winds):
+        for name, v in (("uwind", uw), ("vwind", vw), ("ssh", np.zeros(nx * ny))):
+            g.setData(name, v, 1)
+        g.update(dt)
+        if k + 1 not in ref:
+            continue
+        got = _fields(g, dg, S, lambda x, n: x.get_cg(n), lambda x, n, c: x.getDGData(n, c))
+        errs = {n: rel_l2(got[n], ref[k + 1][n]) for n in got}
+        _say(f"  step {k + 1} rel L2 errors: " + " ".join(f"{n}={e:.1e}" for n, e in errs.items()))
+        if k + 1 == 5:
+            for n, e in errs.items():
+                assert e < 1e-10, (k + 1, n, e)
+        else:
+            for n, e in errs.items():
+                sp = max(rel_l2(p[k + 1][n], ref[k + 1][n]) for p in spread)
+                _say(f"    {n}: device {e:.2e}, perturbed oracle {sp:.2e}")
+                assert e <= 10 * sp + 1e-10, (k + 1, n, e, sp)
+    assert np.abs(g.get_cg("u")).max() > 1e-3  # the cyclone moved the ice
+    assert ref[10]["damage"][:, 0].min() < 0.5  # and broke some of it
+
+
+def test_bbm_state_round_trip_and_mevp_only_facilities():
+    import nextsimdg_amd as nd
+
+    _say("BBM state round trip")
+    nx = ny = 12
+    o, g = _pair(2, 6, nx, ny, box_coords(nx, ny, 5000.0), _box_mask(nx, ny))
+    _set_state(o, g, np.random.default_rng(1), o.S, 6)
+    blob = g.save_state()
+    d = g.getDGData("damage", 6).copy()
+    g.setData("damage", np.ones(nx * ny), 1)
+    g.load_state(blob)
+    np.testing.assert_array_equal(g.getDGData("damage", 6), d)
+    m = nd.MEVPDynamicsKernel(2, 6)
+    m.initialise(box_coords(nx, ny, 5000.0), False, _box_mask(nx, ny), nx, ny)
+    with pytest.raises(nd.NsdgError):
+        m.load_state(blob)  # the other rheology's blob
+    with pytest.raises(nd.NsdgError):
+        m.getDGData("damage", 6)
+    g.begin_iteration(10.0)
+    with pytest.raises(nd.NsdgError):
+        g.momentum_residual()
+    with pytest.raises(nd.NsdgError):
+        g.set_precision(nd.PREC_FP32)
