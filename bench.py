@@ -41,13 +41,13 @@ def byte_model(nx, ny, cg=2, we=8):
     return elem, node, fused
 
 
-def make_kernel(nx, ny, seed, prec, kind="plain", perturb=0.0):
+def make_kernel(nx, ny, seed, prec, kind="plain", perturb=0.0, brittle=False):
     import numpy as np
     import nextsimdg_amd as nd
     from nextsimdg_amd import inputs
 
     s = inputs.benchmark_setup(nx, ny, seed, kind=kind)
-    k = nd.MEVPDynamicsKernel(2, 6)
+    k = nd.BBMDynamicsKernel(2, 6) if brittle else nd.MEVPDynamicsKernel(2, 6)
     k.initialise(s["coords"], False, s["mask"], nx, ny)
     h = s["h"] * (1.0 + perturb * np.random.default_rng(seed + 777).standard_normal(s["h"].size))
     k.setData("hice", h, 1)

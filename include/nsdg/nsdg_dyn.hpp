@@ -1,5 +1,6 @@
 // C++17 wrapper over the C ABI of nsdg_dyn.h with the reference's class and method
-// names (ParametricMesh, DGTransport, MEVPDynamicsKernel), for C++ callers.
+// names (ParametricMesh, DGTransport, DynamicsKernel with MEVPDynamicsKernel and
+// BBMDynamicsKernel), for C++ callers.
 // Status codes become nsdg::Error exceptions; buffers are std::vector<double> in the
 // ModelArray layout. Header-only; link against libnsdg_hip.so.
 #ifndef NSDG_DYN_HPP
@@ -95,18 +96,17 @@ private:
     nsdg_transport* h_ = nullptr;
 };
 
-// MEVPDynamicsKernel<DG> (dynamics/src/include/MEVPDynamicsKernel.hpp:18-37)
-class MEVPDynamicsKernel {
+// DynamicsKernel<DGadvection, DGstress> (dynamics/src/include/DynamicsKernel.hpp:37-229):
+// the interface the module calls, shared by the two rheologies below.
+class DynamicsKernel {
 public:
-    MEVPDynamicsKernel(int cgDegree, int dgAdvection, const nsdg_params& p = defaultParams())
-        : dg_(dgAdvection)
-        , s_(cgDegree == 1 ? 3 : 8)
+    virtual ~DynamicsKernel()
     {
-        check(nsdg_dyn_create(cgDegree, dgAdvection, &p, &h_));
+        if (h_)
+            nsdg_dyn_destroy(h_);
     }
-    ~MEVPDynamicsKernel() { nsdg_dyn_destroy(h_); }
-    MEVPDynamicsKernel(const MEVPDynamicsKernel&) = delete;
-    MEVPDynamicsKernel& operator=(const MEVPDynamicsKernel&) = delete;
+    DynamicsKernel(const DynamicsKernel&) = delete;
+    DynamicsKernel& operator=(const DynamicsKernel&) = delete;
 
     void initialise(size_t nx, size_t ny, const std::vector<double>& coords, bool isSpherical,
         const std::vector<double>& mask)
@@ -137,10 +137,46 @@ public:
     void synchronize() { check(nsdg_dyn_synchronize(h_)); }
     nsdg_dyn* handle() { return h_; }
 
-private:
+protected:
+    DynamicsKernel(int cgDegree, int dgAdvection)
+        : dg_(dgAdvection)
+        , s_(cgDegree == 1 ? 3 : 8)
+    {
+    }
     nsdg_dyn* h_ = nullptr;
+
+private:
     int dg_, s_;
     size_t nx_ = 0, ny_ = 0;
+};
+
+// MEVPDynamicsKernel<DG> (dynamics/src/include/MEVPDynamicsKernel.hpp:18-37)
+class MEVPDynamicsKernel : public DynamicsKernel {
+public:
+    MEVPDynamicsKernel(int cgDegree, int dgAdvection, const nsdg_params& p = defaultParams())
+        : DynamicsKernel(cgDegree, dgAdvection)
+    {
+        check(nsdg_dyn_create(cgDegree, dgAdvection, &p, &h_));
+    }
+};
+
+inline nsdg_meb_params defaultMEBParams()
+{
+    nsdg_meb_params m;
+    check(nsdg_meb_params_default(&m));
+    return m;
+}
+
+// BBMDynamicsKernel<DG> (dynamics/src/include/BBMDynamicsKernel.hpp:18-45); the extra
+// field "damage" goes through setData / getDGData like hice.
+class BBMDynamicsKernel : public DynamicsKernel {
+public:
+    BBMDynamicsKernel(int cgDegree, int dgAdvection, const nsdg_params& p = defaultParams(),
+        const nsdg_meb_params& m = defaultMEBParams())
+        : DynamicsKernel(cgDegree, dgAdvection)
+    {
+        check(nsdg_bbm_create(cgDegree, dgAdvection, &p, &m, &h_));
+    }
 };
 
 } // namespace nsdg

@@ -1050,13 +1050,29 @@ template <int CG> using NodeTile = TileShape<CG, 64, 4>;
 template <int CG> using FusedTile = TileShape<CG, 31, 7>;
 static_assert(FusedTile<2>::HX * FusedTile<2>::HY == kBlock, "fused tile must give one element per thread");
 
-// Phase 2 of a node tile: the owned nodes sum their <= 4 contributions (two-colour stripe
-// order, land contributions exact zeros) and take the FP64 momentum update from the frozen
-// coefficients; u, v are read from a.u, a.v and written to uOut, vOut. Each thread updates
-// up to NB nodes of one column, all their loads issued first.
-template <int CG, class T, class R>
-__device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const R (*contrib)[cgdofs(CG)][2], int ex0, int ey0,
-    double* uOut, double* vOut)
+// Node update policies of nodeTileUpdate: In is what a node loads before its update;
+// kAllNodes runs the update on Dirichlet nodes too (their divergence is zero, their
+// velocity is zeroed after the update).
+// mEVP (VPCGDynamicsKernel.hpp:132-172) from the frozen coefficients; Dirichlet nodes skip it.
+struct MevpNode {
+    using In = NodeFz;
+    static constexpr bool kAllNodes = false;
+    __device__ __forceinline__ void load(const NodeArgs& a, int n, In& in) const { loadFrozen(a, n, in); }
+    __device__ __forceinline__ void update(const NodeArgs& a, const In& in, int, bool, double dx, double dy,
+        double& un, double& vn) const
+    {
+        momentumFrozen(a, in, dx, dy, un, vn);
+    }
+};
+
+// Phase 2 of a node tile: the owned nodes sum their <= 4 contributions (contrib(le, j, c):
+// local element le, its node j, component c; two-colour stripe order, land contributions
+// exact zeros) and take the FP64 momentum update of the policy P; u, v are read from
+// a.u, a.v and written to uOut, vOut. Each thread updates up to NB
+// nodes of one column, all their loads issued first.
+template <int CG, class T, class P = MevpNode, class G>
+__device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const G& contrib, int ex0, int ey0, double* uOut,
+    double* vOut, const P& pol = P {})
 {
     const int row = CG * a.nx + 1;
     const int nx0 = CG * ex0, ny0 = CG * ey0;
@@ -1067,7 +1083,7 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const R (*cont
         return;
     for (int c = threadIdx.x % T::TPR; c < W; c += T::TPR) {
         for (int r0 = threadIdx.x / T::TPR; r0 < H; r0 += NB * T::RS) {
-            NodeFz in[NB];
+            typename P::In in[NB];
             bool live[NB], dir[NB];
             const int nix = nx0 + c;
 #pragma unroll
@@ -1076,8 +1092,8 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const R (*cont
                 live[b] = r < H;
                 const int n = nix + row * (ny0 + r);
                 dir[b] = live[b] ? a.dirNode[n] != 0 : true;
-                if (live[b] && !dir[b])
-                    loadFrozen(a, n, in[b]);
+                if (live[b] && (P::kAllNodes || !dir[b]))
+                    pol.load(a, n, in[b]);
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -1085,15 +1101,16 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const R (*cont
                     continue;
                 const int niy = ny0 + r0 + b * T::RS;
                 double un = 0.0, vn = 0.0;
+                double dx = 0.0, dy = 0.0;
                 if (!dir[b]) {
-                    double dx = 0.0, dy = 0.0;
                     forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
                         const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
-                        dx -= static_cast<double>(contrib[le][j][0]);
-                        dy -= static_cast<double>(contrib[le][j][1]);
+                        dx -= static_cast<double>(contrib(le, j, 0));
+                        dy -= static_cast<double>(contrib(le, j, 1));
                     });
-                    momentumFrozen(a, in[b], dx, dy, un, vn);
                 }
+                if (P::kAllNodes || !dir[b])
+                    pol.update(a, in[b], nix + row * niy, dir[b], dx, dy, un, vn);
                 uOut[nix + row * niy] = un;
                 vOut[nix + row * niy] = vn;
             }
@@ -1134,7 +1151,8 @@ template <int CG> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs
     }
     __syncthreads();
     // Phase 2
-    nodeTileUpdate<CG, T, double>(a, contrib, ex0, ey0, a.u, a.v);
+    nodeTileUpdate<CG, T>(
+        a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, a.u, a.v);
 }
 
 // One whole mEVP subcycle in one pass over a uniform mesh (FP64): per tile, phase 1 runs
@@ -1201,7 +1219,8 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
         }
     }
     __syncthreads();
-    nodeTileUpdate<CG, T, R>(a, contrib, ex0, ey0, f.uOut, f.vOut);
+    nodeTileUpdate<CG, T>(
+        a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
 }
 
 // Split passes (periodic meshes and per-pass testing).
@@ -1535,26 +1554,71 @@ struct BrittleArgs {
     const double* damageIn;
     double* damage; // SoA [DG][ne]
 };
+// The BBM arithmetic at one Gauss point (BBMStressUpdateStep.hpp:67-140): y11, y12, y22
+// hold the interpolated stress on entry and the new one on exit, dg the damage. POW5
+// (exponent_relaxation_sigma 5, exponent_compression_factor 1.5, the MEB defaults) writes
+// the two powers as products, x^4 = (x^2)^2 and h^2.5 = h^2 sqrt(h).
+template <bool POW5 = false>
+__device__ __forceinline__ void bbmPoint(const BrittleArgs& b, double hg, double ag, double& dg, double x11, double x12,
+    double x22, double& y11, double& y12, double& y22)
+{
+    const double nu = b.nu, dt = b.dt;
+    hg = fmax(hg, 0.0);
+    ag = fmin(fmax(ag, 0.0), 1.0);
+    dg = fmin(fmax(dg, 1e-12), 1.0);
+    const double sn0 = 0.5 * (y11 + y22);
+    const double expC = exp(b.compaction * (1.0 - ag));
+    double tv, Pmax;
+    if constexpr (POW5) {
+        const double x = dg * expC, x2 = x * x;
+        tv = b.tau0 * (x2 * x2);
+        Pmax = b.P0 * (hg * hg * sqrt(hg)) * expC;
+    } else {
+        tv = b.tau0 * pow(dg * expC, b.expR);
+        Pmax = b.P0 * pow(hg, b.expP) * expC;
+    }
+    const double tildeP = sn0 < 0.0 ? fmin(-Pmax / sn0, 1.0) : 0.0;
+    const double mult = tv / (tv + (1.0 - tildeP) * dt);
+    const double E = hg * b.young * dg * expC;
+    const double Du = dt * E / (1.0 - nu * nu);
+    y11 = (y11 + Du * (x11 + nu * x22)) * mult;
+    y22 = (y22 + Du * (nu * x11 + x22)) * mult;
+    y12 = (y12 + Du * x12 * (1.0 - nu)) * mult;
+    const double sn = 0.5 * (y11 + y22);
+    const double tau = sqrt(0.25 * (y11 - y22) * (y11 - y22) + y12 * y12);
+    const double cohesion = b.cohesion * hg, cstr = b.compr * hg;
+    double dcrit = (tau + b.tanPhi * sn > 0.0) ? cohesion / (tau + b.tanPhi * sn) : 1.0;
+    dcrit = sn < -cstr ? -cstr / sn : dcrit;
+    dcrit = fmin(dcrit, 1.0);
+    const double td = b.tdNum / sqrt(E);
+    const double relax = (1.0 - dcrit) * dt / td;
+    dg -= dg * relax;
+    y11 -= y11 * relax;
+    y12 -= y12 * relax;
+    y22 -= y22 * relax;
+}
+
 // One element of the BBM stress and damage update. Strain coefficients from the velocity
 // (STRAIN, ocean elements; stored) or the stored ones (land; a stress pass alone); at each Gauss point of the stress
 // rule the elastic increment, the viscous relaxation (tildeP from the compressive limit
 // Pmax), the Mohr-Coulomb / compressive failure d_crit and the damage relaxation over the
 // time t_d; then the stress projected with M^-1 PSI diag(w J) from the operator table and
 // the damage with psi_d w / m_d (exact for the affine elements of a uniform mesh).
-template <int CG, int DG, bool STRAIN>
-__global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs b)
+// The new stress goes to acc1..3 and the new damage to accD; with STORE_STRAIN the strain
+// of an ocean element is stored as well.
+template <int CG, int DG, bool STRAIN, bool STORE_STRAIN>
+__device__ __forceinline__ void bbmElement(const ElemArgs& a, const BrittleArgs& b, int e, double* acc1,
+    double* acc2, double* acc3, double* accD)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, Q = L::Q;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.ne)
-        return;
     const size_t ne = a.ne;
     const Ops<true> op { a.ops, a.stride };
     double e11[S], e12[S], e22[S];
     if (STRAIN && a.ocean[e]) {
         strainLocal<CG, true>(a, e, e11, e12, e22);
-        storeTensor<S>(a.epsOut, ne, e, e11, e12, e22);
+        if constexpr (STORE_STRAIN)
+            storeTensor<S>(a.epsOut, ne, e, e11, e12, e22);
     } else {
         loadTensor<S>(a.eps, ne, e, e11, e12, e22);
     }
@@ -1566,14 +1630,12 @@ __global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs 
         ac[i] = a.A[i * ne + e];
         dc[i] = b.damageIn[i * ne + e];
     }
-    double acc1[S], acc2[S], acc3[S], accD[DG];
 #pragma unroll
     for (int k = 0; k < S; ++k)
         acc1[k] = acc2[k] = acc3[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < DG; ++k)
         accD[k] = 0.0;
-    const double nu = b.nu, dt = b.dt;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         double hg = 0, ag = 0, dg = 0;
@@ -1594,32 +1656,7 @@ __global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs 
             y12 += s12[k] * ps;
             y22 += s22[k] * ps;
         }
-        hg = fmax(hg, 0.0);
-        ag = fmin(fmax(ag, 0.0), 1.0);
-        dg = fmin(fmax(dg, 1e-12), 1.0);
-        const double sn0 = 0.5 * (y11 + y22);
-        const double expC = exp(b.compaction * (1.0 - ag));
-        const double tv = b.tau0 * pow(dg * expC, b.expR);
-        const double Pmax = b.P0 * pow(hg, b.expP) * expC;
-        const double tildeP = sn0 < 0.0 ? fmin(-Pmax / sn0, 1.0) : 0.0;
-        const double mult = tv / (tv + (1.0 - tildeP) * dt);
-        const double E = hg * b.young * dg * expC;
-        const double Du = dt * E / (1.0 - nu * nu);
-        y11 = (y11 + Du * (x11 + nu * x22)) * mult;
-        y22 = (y22 + Du * (nu * x11 + x22)) * mult;
-        y12 = (y12 + Du * x12 * (1.0 - nu)) * mult;
-        const double sn = 0.5 * (y11 + y22);
-        const double tau = sqrt(0.25 * (y11 - y22) * (y11 - y22) + y12 * y12);
-        const double cohesion = b.cohesion * hg, cstr = b.compr * hg;
-        double dcrit = (tau + b.tanPhi * sn > 0.0) ? cohesion / (tau + b.tanPhi * sn) : 1.0;
-        dcrit = sn < -cstr ? -cstr / sn : dcrit;
-        dcrit = fmin(dcrit, 1.0);
-        const double td = b.tdNum / sqrt(E);
-        const double relax = (1.0 - dcrit) * dt / td;
-        dg -= dg * relax;
-        y11 -= y11 * relax;
-        y12 -= y12 * relax;
-        y22 -= y22 * relax;
+        bbmPoint(b, hg, ag, dg, x11, x12, x22, y11, y12, y22);
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const double w = op(L::B + k * Q + q, e);
@@ -1633,40 +1670,295 @@ __global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs 
                            * invMassRef(k))
                 * dg;
     }
-    storeTensor<S>(a.sig, ne, e, acc1, acc2, acc3);
+}
+// Factors of the DG basis (dgBasisAt): psi_k(X, Y) = f_FX[k](X) g_GY[k](Y) with f, g in
+// {1, t, t^2 - 1/12}; on a Gauss row (fixed Y) a DG field is r0 + r1 X + r2 (X^2 - 1/12).
+__device__ __forceinline__ constexpr int basisFX(int k) { return k == 1 || k == 5 || k == 7 ? 1 : (k == 3 || k == 6 ? 2 : 0); }
+__device__ __forceinline__ constexpr int basisGY(int k) { return k == 2 || k == 5 || k == 6 ? 1 : (k == 4 || k == 7 ? 2 : 0); }
+__device__ __forceinline__ constexpr double legendre01(int i, double t) { return i == 0 ? 1.0 : (i == 1 ? t : t * t - 1.0 / 12.0); }
+
+// bbmElement for the one-pass kernel on an affine uniform mesh, with the operators of
+// elementAffineCompute: the Gauss-point strain is the exact gradient of the CG velocity by
+// sum factorisation (ocean elements; land elements interpolate their stored strain), both
+// projections are psi_k(q) w_q / m_k literals, and every DG field is evaluated per Gauss
+// row from its three row coefficients. The previous stress waits in the thread's own
+// column of stage (LDS, [3 S][kBlock]) instead of 3 S registers. The strain is not stored.
+template <int CG, int DG, bool POW5>
+__device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const BrittleArgs& b, int e, double (*stage)[kBlock],
+    double* acc1, double* acc2, double* acc3, double* accD)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
+    const size_t ne = a.ne;
+    const int t = threadIdx.x;
+    const bool oc = a.ocean[e] != 0;
+#pragma unroll
+    for (int k = 0; k < 3 * S; ++k)
+        stage[k][t] = a.sig[k * ne + e];
+    double buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
+    if (oc) {
+        const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+        const int cgi = CG * row * iy + CG * ix;
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                buf[c + K * r] = a.u[cgi + c + r * row];
+                buf[N + c + K * r] = a.v[cgi + c + r * row];
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3 * S; ++k)
+            buf[k] = a.eps[k * ne + e];
+    }
+    double hc[DG], ac[DG], dc[DG];
+#pragma unroll
+    for (int i = 0; i < DG; ++i) {
+        hc[i] = a.h[i * ne + e];
+        ac[i] = a.A[i * ne + e];
+        dc[i] = b.damageIn[i * ne + e];
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        acc1[k] = acc2[k] = acc3[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < DG; ++k)
-        b.damage[k * ne + e] = accD[k];
+        accD[k] = 0.0;
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy) {
+        const double Y = gauss_point(NG, gy) - 0.5;
+        // row coefficients of h, A, damage and the three stress components
+        double rh[3] = { 0, 0, 0 }, ra[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 };
+        double r1[3] = { 0, 0, 0 }, r2[3] = { 0, 0, 0 }, r3[3] = { 0, 0, 0 };
+#pragma unroll
+        for (int i = 0; i < DG; ++i) {
+            const double g = legendre01(basisGY(i), Y);
+            rh[basisFX(i)] += hc[i] * g;
+            ra[basisFX(i)] += ac[i] * g;
+            rd[basisFX(i)] += dc[i] * g;
+        }
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double g = legendre01(basisGY(k), Y);
+            r1[basisFX(k)] += stage[k][t] * g;
+            r2[basisFX(k)] += stage[S + k][t] * g;
+            r3[basisFX(k)] += stage[2 * S + k][t] * g;
+        }
+        double Iu[K], Du[K], Iv[K], Dv[K];
+        if (oc) {
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                double iu = 0, du = 0, iv = 0, dv = 0;
+#pragma unroll
+                for (int jy = 0; jy < K; ++jy) {
+                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+                    iu += ph * buf[jx + K * jy];
+                    du += dp * buf[jx + K * jy];
+                    iv += ph * buf[N + jx + K * jy];
+                    dv += dp * buf[N + jx + K * jy];
+                }
+                Iu[jx] = iu;
+                Du[jx] = du;
+                Iv[jx] = iv;
+                Dv[jx] = dv;
+            }
+        }
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            const double X = gauss_point(NG, gx) - 0.5, X2 = X * X - 1.0 / 12.0;
+            double bd[8];
+            dgBasisAt(X, Y, bd);
+            double g11 = 0, g12 = 0, g22 = 0;
+            if (oc) {
+                double uxi = 0, ueta = 0, vxi = 0, veta = 0;
+#pragma unroll
+                for (int jx = 0; jx < K; ++jx) {
+                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                    uxi += dp * Iu[jx];
+                    ueta += ph * Du[jx];
+                    vxi += dp * Iv[jx];
+                    veta += ph * Dv[jx];
+                }
+                g11 = uxi * a.xiX + ueta * a.etaX;
+                g22 = vxi * a.xiY + veta * a.etaY;
+                g12 = 0.5 * ((uxi * a.xiY + ueta * a.etaY) + (vxi * a.xiX + veta * a.etaX));
+            } else {
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    g11 += buf[k] * bd[k];
+                    g12 += buf[S + k] * bd[k];
+                    g22 += buf[2 * S + k] * bd[k];
+                }
+            }
+            double hg = rh[0] + rh[1] * X + rh[2] * X2, ag = ra[0] + ra[1] * X + ra[2] * X2;
+            double dg = rd[0] + rd[1] * X + rd[2] * X2;
+            double y11 = r1[0] + r1[1] * X + r1[2] * X2, y12 = r2[0] + r2[1] * X + r2[2] * X2;
+            double y22 = r3[0] + r3[1] * X + r3[2] * X2;
+            bbmPoint<POW5>(b, hg, ag, dg, g11, g12, g22, y11, y12, y22);
+            const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy);
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const double c = bd[k] * w * invMassRef(k);
+                if (c == 0.0) // basis functions vanishing at this Gauss point (a literal test)
+                    continue;
+                acc1[k] += c * y11;
+                acc2[k] += c * y12;
+                acc3[k] += c * y22;
+            }
+#pragma unroll
+            for (int k = 0; k < DG; ++k) {
+                const double c = bd[k] * w * invMassRef(k);
+                if (c == 0.0)
+                    continue;
+                accD[k] += c * dg;
+            }
+        }
+    }
+}
+
+template <int CG, int DG, bool STRAIN>
+__global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs b)
+{
+    constexpr int S = cg2dgstress(CG);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne)
+        return;
+    double acc1[S], acc2[S], acc3[S], accD[DG];
+    bbmElement<CG, DG, STRAIN, STRAIN>(a, b, e, acc1, acc2, acc3, accD);
+    storeTensor<S>(a.sig, a.ne, e, acc1, acc2, acc3);
+#pragma unroll
+    for (int k = 0; k < DG; ++k)
+        b.damage[static_cast<size_t>(k) * a.ne + e] = accD[k];
 }
 
 // BrittleCGDynamicsKernel::updateMomentum (BrittleCGDynamicsKernel.hpp:200-253) for one
 // node: the semi-implicit substep with the Coriolis term and the ocean drag turned by the
 // turning angle, solved as the reference writes it (its v carries +beta (gradX + tauX)),
 // and the step-averaged velocity accumulated before the Dirichlet zero.
-__global__ __launch_bounds__(kBlock) void k_brittle_momentum(NodeArgs a, double ct, double st, double nsteps,
-    double* avgU, double* avgV)
+struct BrittleIn {
+    double u, v, H, A, uo, vo, ua, va, gx, gy, ML, au, av;
+};
+struct BrittleNode {
+    using In = BrittleIn;
+    static constexpr bool kAllNodes = true;
+    double ct, st, nsteps;
+    double* avgU;
+    double* avgV;
+    __device__ __forceinline__ void load(const NodeArgs& a, int n, In& in) const
+    {
+        in.u = a.u[n];
+        in.v = a.v[n];
+        in.H = a.H[n];
+        in.A = a.A[n];
+        in.uo = a.uo[n];
+        in.vo = a.vo[n];
+        in.ua = a.ua[n];
+        in.va = a.va[n];
+        in.gx = a.gx[n];
+        in.gy = a.gy[n];
+        in.ML = a.ML[n];
+        in.au = avgU[n];
+        in.av = avgV[n];
+    }
+    // dSx, dSy are zero on Dirichlet nodes (dirichletZero of the divergence)
+    __device__ __forceinline__ void update(const NodeArgs& a, const In& in, int n, bool dir, double dSx, double dSy,
+        double& un, double& vn) const
+    {
+        const double dtm = a.dt / (a.rho_ice * in.H);
+        const double cP = in.A * a.F_ocean * hypot(in.uo - in.u, in.vo - in.v);
+        const double al = 1.0 + dtm * (cP * ct);
+        const double be = a.dt * a.fc + dtm * cP * st;
+        const double rD = 1.0 / (al * al + be * be);
+        const double dA = in.A * a.F_atm * hypot(in.ua, in.va);
+        const double tX = dA * in.ua + cP * (in.uo * ct - in.vo * st);
+        const double tY = dA * in.va + cP * (in.vo * ct + in.uo * st);
+        const double gX = dSx / in.ML - a.rho_ice * in.H * a.gravity * in.gx;
+        const double gY = dSy / in.ML - a.rho_ice * in.H * a.gravity * in.gy;
+        un = (al * in.u + be * in.v + dtm * (al * (gX + tX) + be * (gY + tY))) * rD;
+        vn = (al * in.v - be * in.u + dtm * (al * (gY + tY) + be * (gX + tX))) * rD;
+        avgU[n] = in.au + un / nsteps;
+        avgV[n] = in.av + vn / nsteps;
+        if (dir) {
+            un = 0.0;
+            vn = 0.0;
+        }
+    }
+};
+// Split pass: every node, the Dirichlet zero left to k_boundaries.
+__global__ __launch_bounds__(kBlock) void k_brittle_momentum(NodeArgs a, BrittleNode pol)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.nn)
         return;
-    const double H = a.H[n], Ac = a.A[n], uo = a.uo[n], vo = a.vo[n], ua = a.ua[n], va = a.va[n];
-    const double uI = a.u[n], vI = a.v[n];
-    const double dtm = a.dt / (a.rho_ice * H);
-    const double cP = Ac * a.F_ocean * hypot(uo - uI, vo - vI);
-    const double al = 1.0 + dtm * (cP * ct);
-    const double be = a.dt * a.fc + dtm * cP * st;
-    const double rD = 1.0 / (al * al + be * be);
-    const double dA = Ac * a.F_atm * hypot(ua, va);
-    const double tX = dA * ua + cP * (uo * ct - vo * st);
-    const double tY = dA * va + cP * (vo * ct + uo * st);
-    const double gX = a.dSx[n] / a.ML[n] - a.rho_ice * H * a.gravity * a.gx[n];
-    const double gY = a.dSy[n] / a.ML[n] - a.rho_ice * H * a.gravity * a.gy[n];
-    const double un = (al * uI + be * vI + dtm * (al * (gX + tX) + be * (gY + tY))) * rD;
-    const double vn = (al * vI - be * uI + dtm * (al * (gY + tY) + be * (gX + tX))) * rD;
+    BrittleIn in;
+    pol.load(a, n, in);
+    double un, vn;
+    pol.update(a, in, n, false, a.dSx[n], a.dSy[n], un, vn);
     a.u[n] = un;
     a.v[n] = vn;
-    avgU[n] += un / nsteps;
-    avgV[n] += vn / nsteps;
+}
+
+// One whole brittle substep in one pass over a uniform mesh without periodic segments,
+// laid out as k_mevp_fused: phase 1 updates stress and damage of the tile and its
+// left/bottom halo in registers (stores for the tile's own elements), phase 2 sums the
+// weak divergence from LDS and runs the brittle node update. Stress, damage and
+// velocities are ping-ponged; the averaged velocity is updated in place by the node's
+// owner. The arithmetic is that of k_bbm_element, k_divergence, k_brittle_momentum and
+// k_boundaries in sequence (the strain is not stored).
+struct BrittleFusedArgs {
+    ElemArgs e;
+    NodeArgs n;
+    BrittleArgs b;
+    BrittleNode pol;
+    double* sigOut;
+    double* damageOut;
+    double* uOut;
+    double* vOut;
+};
+template <int CG, int DG, bool POW5> __global__ __launch_bounds__(kBlock) void k_bbm_fused(BrittleFusedArgs f)
+{
+    using T = FusedTile<CG>;
+    constexpr int N = T::N, S = cg2dgstress(CG);
+    static_assert(3 * S >= 2 * N, "the stress stage holds the divergence contributions");
+    // column t: thread t's previous stress in phase 1, then its element's contributions
+    // (row 2 j + c: node j, component c), which phase 2 reads across columns
+    __shared__ double stage[3 * S][kBlock];
+    const NodeArgs& a = f.n;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+    {
+        const int le = threadIdx.x; // one element of tile + halo per thread
+        const int lx = le % T::HX, ly = le / T::HX;
+        const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
+        double tx[N], ty[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            tx[j] = ty[j] = 0;
+        if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
+            const int e = ex + a.nx * ey;
+            double s11[S], s12[S], s22[S], d[DG];
+            bbmElementAffine<CG, DG, POW5>(f.e, f.b, e, stage, s11, s12, s22, d);
+            if (lx >= 1 && ly >= 1) {
+                storeTensor<S>(f.sigOut, a.ne, e, s11, s12, s22);
+#pragma unroll
+                for (int k = 0; k < DG; ++k)
+                    f.damageOut[static_cast<size_t>(k) * a.ne + e] = d[k];
+            }
+            if (a.ocean[e])
+                divergenceAffineFrom<CG, double>(a, s11, s12, s22, tx, ty);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            stage[2 * j][le] = tx[j];
+            stage[2 * j + 1][le] = ty[j];
+        }
+    }
+    __syncthreads();
+    nodeTileUpdate<CG, T, BrittleNode>(
+        a, [&](int le, int j, int c) { return stage[2 * j + c][le]; }, ex0, ey0, f.uOut, f.vOut, f.pol);
 }
 
 // BrittleCGDynamicsKernel::getIceOceanStress (BrittleCGDynamicsKernel.hpp:163-182): from
@@ -1713,7 +2005,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // stress transport in the stress space, element size of the uniform mesh
     bool brittle = false;
     nsdg_meb_params meb {};
-    DBuf<double> damage, avgU, avgV;
+    DBuf<double> damage, damageAlt, avgU, avgV;
     std::unique_ptr<TransportBase> stressTr;
     CG2DGOps cg2dgS;
     double hElem = 0.0;
@@ -1832,6 +2124,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         stressTr->setScheme(2);
         cg2dgS.build(CG, S, mesh, s);
         damage.alloc(ne * DG);
+        damageAlt.alloc(ne * DG);
         deviceFill(damage.p, ne * DG, 0.0, s);
         deviceFill(damage.p, ne, 1.0, s);
         avgU.alloc(nn);
@@ -2347,20 +2640,55 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         k_bbm_element<CG, DG, STRAIN><<<blocksFor(ne), kBlock, 0, s>>>(elemArgs(), brittleArgs());
         checkLaunch("bbm_element");
     }
+    BrittleNode brittleNode() const
+    {
+        return BrittleNode { turnCos(), turnSin(), static_cast<double>(p.nsteps), avgU.p, avgV.p };
+    }
     void brittleMomentumAsync()
     {
         Timed t(prof, s, "brittle_momentum");
-        k_brittle_momentum<<<blocksFor(nn), kBlock, 0, s>>>(
-            nodeArgs(), turnCos(), turnSin(), static_cast<double>(p.nsteps), avgU.p, avgV.p);
+        k_brittle_momentum<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), brittleNode());
         checkLaunch("brittle_momentum");
     }
+    void brittleFusedPass()
+    {
+        Timed t(prof, s, "bbm_fused");
+        BrittleFusedArgs f {};
+        f.e = elemArgs();
+        f.n = nodeArgs();
+        f.b = brittleArgs();
+        f.pol = brittleNode();
+        f.sigOut = sigAlt.p;
+        f.damageOut = damageAlt.p;
+        f.uOut = uAlt.p;
+        f.vOut = vAlt.p;
+        using T = FusedTile<CG>;
+        const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+        if (meb.exponent_relaxation_sigma == 5.0 && meb.exponent_compression_factor == 1.5)
+            k_bbm_fused<CG, DG, true><<<grid, kBlock, 0, s>>>(f);
+        else
+            k_bbm_fused<CG, DG, false><<<grid, kBlock, 0, s>>>(f);
+        checkLaunch("bbm_fused");
+        swapBuf(sig, sigAlt);
+        swapBuf(damage, damageAlt);
+        swapBuf(u, uAlt);
+        swapBuf(v, vAlt);
+    }
+    // Substeps: one pass each on uniform meshes without periodic segments (the strain of
+    // the last substep's velocity stored before it), else the reference's pass sequence.
     void brittleSubstepsAsync(int n)
     {
         for (int k = 0; k < n; ++k) {
-            brittleStressAsync<true>();
-            divergenceAsync();
-            brittleMomentumAsync();
-            boundariesAsync();
+            if (fusedSubcycle()) {
+                if (k == n - 1)
+                    strainAsync();
+                brittleFusedPass();
+            } else {
+                brittleStressAsync<true>();
+                divergenceAsync();
+                brittleMomentumAsync();
+                boundariesAsync();
+            }
         }
     }
     // Advection of h and A with the averaged velocity, transport of the stress (stress

@@ -1,15 +1,19 @@
 // C++ caller of the gfx950 library through include/nsdg/nsdg_dyn.hpp: one module-level
-// step sequence of a small benchmark box; prints the cell means of u, v, hice, cice.
+// step sequence of a small benchmark box; prints the sums of the cell means of u, v, hice,
+// cice (and damage with the brittle rheology). Usage: wrapper_demo [steps] [mevp|bbm]
 #include "nsdg/nsdg_dyn.hpp"
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <memory>
 #include <vector>
 
 int main(int argc, char** argv)
 {
     const size_t nx = 24, ny = 24;
     const int steps = argc > 1 ? std::atoi(argv[1]) : 2;
+    const bool brittle = argc > 2 && std::strcmp(argv[2], "bbm") == 0;
     const double L = 512e3, dx = L / nx;
     std::vector<double> coords(2 * (nx + 1) * (ny + 1)), mask(nx * ny, 1.0), h(nx * ny), a(nx * ny, 1.0),
         uo(nx * ny), vo(nx * ny), zero(nx * ny, 0.0);
@@ -28,7 +32,12 @@ int main(int argc, char** argv)
             vo[e] = 0.01 * (1.0 - 2.0 * i / nx);
         }
     try {
-        nsdg::MEVPDynamicsKernel k(2, 6);
+        std::unique_ptr<nsdg::DynamicsKernel> kp;
+        if (brittle)
+            kp = std::make_unique<nsdg::BBMDynamicsKernel>(2, 6);
+        else
+            kp = std::make_unique<nsdg::MEVPDynamicsKernel>(2, 6);
+        nsdg::DynamicsKernel& k = *kp;
         k.initialise(nx, ny, coords, false, mask);
         k.setData("uocean", uo);
         k.setData("vocean", vo);
@@ -49,7 +58,14 @@ int main(int argc, char** argv)
             sh += h[e];
             sa += a[e];
         }
-        std::printf("%.17g %.17g %.17g %.17g\n", su, sv, sh, sa);
+        std::printf("%.17g %.17g %.17g %.17g", su, sv, sh, sa);
+        if (brittle) {
+            double sd = 0;
+            for (double x : k.getDG0Data("damage"))
+                sd += x;
+            std::printf(" %.17g", sd);
+        }
+        std::printf("\n");
     } catch (const nsdg::Error& e) {
         std::fprintf(stderr, "nsdg error %d: %s\n", e.code, e.what());
         return 1;

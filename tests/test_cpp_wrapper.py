@@ -23,12 +23,14 @@ def test_cpp_wrapper_compiles_and_links():
 
 
 @pytest.mark.gpu
-def test_cpp_wrapper_run_matches_oracle():
+@pytest.mark.parametrize("rheology", ["mevp", "bbm"])
+def test_cpp_wrapper_run_matches_oracle(rheology):
     from oracle import Oracle, OracleDyn
 
-    out = subprocess.run([build_demo(), "2"], capture_output=True, text=True, timeout=120)
+    print(f"C++ wrapper run, {rheology}", flush=True)
+    out = subprocess.run([build_demo(), "2", rheology], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
-    su, sv, sh, sa = map(float, out.stdout.split())
+    got = list(map(float, out.stdout.split()))
     nx = ny = 24
     L, dx = 512e3, 512e3 / 24
     j, i = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
@@ -38,7 +40,7 @@ def test_cpp_wrapper_run_matches_oracle():
     mask = np.where((ic == 0) | (jc == 0) | (ic == nx - 1) | (jc == ny - 1), 0.0, 1.0)
     h = 0.3 + 0.005 * (np.sin(6e-5 * ic * dx) + np.sin(3e-5 * jc * dx))
     a = np.ones(nx * ny)
-    o = OracleDyn(2, 6)
+    o = OracleDyn(2, 6, bbm=rheology == "bbm")
     o.initialise(nx, ny, coords, mask)
     o.set_field("uocean", 0.01 * (2.0 * jc / ny - 1.0), 1)
     o.set_field("vocean", 0.01 * (1.0 - 2.0 * ic / nx), 1)
@@ -53,5 +55,8 @@ def test_cpp_wrapper_run_matches_oracle():
         o.update(120.0)
         h, a = o.get_field("hice", 1), o.get_field("cice", 1)
     ref = [o.get_field("u", 1).sum(), o.get_field("v", 1).sum(), h.sum(), a.sum()]
-    for got, want in zip((su, sv, sh, sa), ref):
-        assert got == pytest.approx(want, rel=1e-9, abs=1e-12)
+    if rheology == "bbm":
+        ref.append(o.get_field("damage", 1).sum())
+    assert len(got) == len(ref)
+    for g, want in zip(got, ref):
+        assert g == pytest.approx(want, rel=1e-9, abs=1e-12)
