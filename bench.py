@@ -86,6 +86,64 @@ def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0, kind="plain"):
     return el, stats
 
 
+def brittle_byte_model(nx, ny, cg=2, dg=6):
+    """Algorithmic HBM bytes of one one-pass brittle substep: stress read + written (3 S
+    coefficients each), damage read + written, h and A read (D coefficients each) per
+    element; per node u, v read + written, the 9 forcing and geometry values (H, A, ocean
+    and wind velocity, SSH gradient, lumped mass) read, the averaged velocity read + written."""
+    S = 8 if cg == 2 else 3
+    ne, nn = nx * ny, (cg * nx + 1) * (cg * ny + 1)
+    return 8 * (6 * S + 4 * dg) * ne + 8 * 17 * nn
+
+
+def brittle_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None):
+    """The BBM (brittle) dynamics step on the same box and forcing: advection, stress and
+    damage transport, 100 substeps of dt / 100. Mcell-BBM-substeps/s and the one-pass
+    substep kernel against its byte model."""
+    k, off, L = make_kernel(nx, nx, seed, 0, brittle=True)
+    run_steps(k, off, L, 0, warmup, dt)
+    k.synchronize()
+    t = time.perf_counter()
+    run_steps(k, off, L, warmup, steps, dt)
+    k.synchronize()
+    el = time.perf_counter() - t
+    k.set_profiling(True)
+    run_steps(k, off, L, warmup + steps, 1, dt)
+    k.synchronize()
+    names = ("bbm_fused", "strain", "transport_op", "limit", "dg2cg", "cg2dg", "ssh_gradient")
+    stats = {n: k.kernel_stats(n) for n in names}
+    k.set_profiling(False)
+    c, tot = stats["bbm_fused"]
+    avg_ms = tot / c
+    alg = brittle_byte_model(nx, nx)
+    return {
+        "workload": f"BBM brittle dynamics step, benchmark box {nx}x{nx}, CG2 velocity, DG2 advection, "
+                    f"8-dof DG stress, FP64, dt=120 s, 100 substeps of 1.2 s, moving cyclone",
+        "value": round(nx * nx * 100 * steps / el / 1e6, 3),
+        "unit": "Mcell-BBM-substeps/s",
+        "steps": steps,
+        "ms_per_step": round(el / steps * 1e3, 4),
+        "kernel": "bbm_fused",
+        "avg_launch_ms": round(avg_ms, 5),
+        "algorithmic_bytes_per_launch": alg,
+        "achieved": round(alg / (avg_ms * 1e-3) / 1e9, 1),
+        "unit_achieved": "GB/s",
+        "hbm_frac": round(alg / (avg_ms * 1e-3) / HBM_PEAK, 4),
+        "traffic": _traffic(traffic_path, f"bbm_fused_{nx}"),
+        "kernels_ms": {n: (round(v[1] / v[0], 5) if v[0] else None) for n, v in stats.items()},
+    }
+
+
+def _traffic(path, key):
+    """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    if not path or not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(key)
+    except (ValueError, OSError):
+        return None
+
+
 def fp32_variant(nx, steps, warmup, err_steps, seed=0):
     """The FP32 variant on heterogeneous ice (h = 0.3 + LogNormal(0, 0.3), A ~ Beta(8, 1)):
     throughput next to the FP64 path on the same inputs, and the per-field relative L2
@@ -231,6 +289,7 @@ def main():
     ap.add_argument("--fp32-nx", type=int, default=1024, help="FP32 variant size (0 skips it)")
     ap.add_argument("--fp32-error-steps", type=int, default=100)
     ap.add_argument("--learned-nx", type=int, default=1024, help="learned vs plain size (0 skips it)")
+    ap.add_argument("--bbm-nx", type=int, default=512, help="brittle (BBM) step size (0 skips it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -262,12 +321,7 @@ def main():
     avg_ms = rstats[dom][1] / rstats[dom][0]
     alg = alg_of[dom]
     achieved = alg / (avg_ms * 1e-3)
-    traffic = None
-    if os.path.exists(args.traffic):
-        try:
-            traffic = json.load(open(args.traffic)).get(f"{dom}_{rnx}")
-        except (ValueError, OSError):
-            traffic = None
+    traffic = _traffic(args.traffic, f"{dom}_{rnx}")
     # % of HBM peak on the two-pass byte model of the reference's formulation (element pass:
     # u, v, h, A, sigma r+w; node pass: sigma and 13 node fields r, u, v w), 8 (9 S + 2 D) per
     # element plus 8 * 17 per node, FP64: the figure whole-step throughput is judged by.
@@ -306,6 +360,9 @@ def main():
         log(f"bench: learned warm start vs plain at {args.learned_nx}^2 ...")
         out["learned"] = learned_vs_plain(args.learned_nx)
         out["learned_vs_plain_speedup_at_equal_residual"] = out["learned"].get("speedup_min")
+    if args.bbm_nx:
+        log(f"bench: BBM brittle step at {args.bbm_nx}^2 ...")
+        out["bbm"] = brittle_leg(args.bbm_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:

@@ -1550,7 +1550,9 @@ __global__ void k_mass_final(const double* part, int nb, double* out)
 // for one substep on a uniform mesh: the element size h_e enters through sqrt(0.1 / h_e)
 // (cohesion and compressive strength) and h_e sqrt(2 (1 + nu) rho_ice) (damage time).
 struct BrittleArgs {
-    double dt, nu, young, P0, expP, compaction, tau0, expR, tanPhi, cohesion, compr, tdNum;
+    double dt, nu, young, P0, expP, compaction, tau0, expR, tanPhi, cohesion, compr;
+    double duFac;  // dt / (1 - nu^2): the elastic increment is duFac E strain
+    double dtOverT; // dt / (h_e sqrt(2 (1 + nu) rho_ice)): dt / t_d = dtOverT sqrt(E)
     const double* damageIn;
     double* damage; // SoA [DG][ne]
 };
@@ -1580,7 +1582,7 @@ __device__ __forceinline__ void bbmPoint(const BrittleArgs& b, double hg, double
     const double tildeP = sn0 < 0.0 ? fmin(-Pmax / sn0, 1.0) : 0.0;
     const double mult = tv / (tv + (1.0 - tildeP) * dt);
     const double E = hg * b.young * dg * expC;
-    const double Du = dt * E / (1.0 - nu * nu);
+    const double Du = b.duFac * E;
     y11 = (y11 + Du * (x11 + nu * x22)) * mult;
     y22 = (y22 + Du * (nu * x11 + x22)) * mult;
     y12 = (y12 + Du * x12 * (1.0 - nu)) * mult;
@@ -1590,8 +1592,7 @@ __device__ __forceinline__ void bbmPoint(const BrittleArgs& b, double hg, double
     double dcrit = (tau + b.tanPhi * sn > 0.0) ? cohesion / (tau + b.tanPhi * sn) : 1.0;
     dcrit = sn < -cstr ? -cstr / sn : dcrit;
     dcrit = fmin(dcrit, 1.0);
-    const double td = b.tdNum / sqrt(E);
-    const double relax = (1.0 - dcrit) * dt / td;
+    const double relax = (1.0 - dcrit) * (b.dtOverT * sqrt(E));
     dg -= dg * relax;
     y11 -= y11 * relax;
     y12 -= y12 * relax;
@@ -1681,20 +1682,15 @@ __device__ __forceinline__ constexpr double legendre01(int i, double t) { return
 // elementAffineCompute: the Gauss-point strain is the exact gradient of the CG velocity by
 // sum factorisation (ocean elements; land elements interpolate their stored strain), both
 // projections are psi_k(q) w_q / m_k literals, and every DG field is evaluated per Gauss
-// row from its three row coefficients. The previous stress waits in the thread's own
-// column of stage (LDS, [3 S][kBlock]) instead of 3 S registers. The strain is not stored.
+// row from its three row coefficients. The strain is not stored.
 template <int CG, int DG, bool POW5>
-__device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const BrittleArgs& b, int e, double (*stage)[kBlock],
-    double* acc1, double* acc2, double* acc3, double* accD)
+__device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const BrittleArgs& b, int e, double* acc1,
+    double* acc2, double* acc3, double* accD)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
     const size_t ne = a.ne;
-    const int t = threadIdx.x;
     const bool oc = a.ocean[e] != 0;
-#pragma unroll
-    for (int k = 0; k < 3 * S; ++k)
-        stage[k][t] = a.sig[k * ne + e];
     double buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
     if (oc) {
         const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
@@ -1711,13 +1707,14 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
         for (int k = 0; k < 3 * S; ++k)
             buf[k] = a.eps[k * ne + e];
     }
-    double hc[DG], ac[DG], dc[DG];
+    double hc[DG], ac[DG], dc[DG], s11[S], s12[S], s22[S];
 #pragma unroll
     for (int i = 0; i < DG; ++i) {
         hc[i] = a.h[i * ne + e];
         ac[i] = a.A[i * ne + e];
         dc[i] = b.damageIn[i * ne + e];
     }
+    loadTensor<S>(a.sig, ne, e, s11, s12, s22);
 #pragma unroll
     for (int k = 0; k < S; ++k)
         acc1[k] = acc2[k] = acc3[k] = 0.0;
@@ -1740,9 +1737,9 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const double g = legendre01(basisGY(k), Y);
-            r1[basisFX(k)] += stage[k][t] * g;
-            r2[basisFX(k)] += stage[S + k][t] * g;
-            r3[basisFX(k)] += stage[2 * S + k][t] * g;
+            r1[basisFX(k)] += s11[k] * g;
+            r2[basisFX(k)] += s12[k] * g;
+            r3[basisFX(k)] += s22[k] * g;
         }
         double Iu[K], Du[K], Iv[K], Dv[K];
         if (oc) {
@@ -1919,10 +1916,8 @@ template <int CG, int DG, bool POW5> __global__ __launch_bounds__(kBlock) void k
 {
     using T = FusedTile<CG>;
     constexpr int N = T::N, S = cg2dgstress(CG);
-    static_assert(3 * S >= 2 * N, "the stress stage holds the divergence contributions");
-    // column t: thread t's previous stress in phase 1, then its element's contributions
-    // (row 2 j + c: node j, component c), which phase 2 reads across columns
-    __shared__ double stage[3 * S][kBlock];
+    // divergence contributions, row 2 j + c (node j, component c), column = local element
+    __shared__ double stage[2 * N][kBlock];
     const NodeArgs& a = f.n;
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
     const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
@@ -1940,7 +1935,7 @@ template <int CG, int DG, bool POW5> __global__ __launch_bounds__(kBlock) void k
         if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
             const int e = ex + a.nx * ey;
             double s11[S], s12[S], s22[S], d[DG];
-            bbmElementAffine<CG, DG, POW5>(f.e, f.b, e, stage, s11, s12, s22, d);
+            bbmElementAffine<CG, DG, POW5>(f.e, f.b, e, s11, s12, s22, d);
             if (lx >= 1 && ly >= 1) {
                 storeTensor<S>(f.sigOut, a.ne, e, s11, s12, s22);
 #pragma unroll
@@ -2629,7 +2624,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         const double scale = std::sqrt(0.1 / hElem);
         b.cohesion = meb.C_lab * scale;
         b.compr = meb.compr_strength * scale;
-        b.tdNum = hElem * std::sqrt(2.0 * (1.0 + meb.nu0) * p.rho_ice);
+        b.duFac = deltaT / (1.0 - meb.nu0 * meb.nu0);
+        b.dtOverT = deltaT / (hElem * std::sqrt(2.0 * (1.0 + meb.nu0) * p.rho_ice));
         b.damageIn = damage.p;
         b.damage = damage.p;
         return b;
