@@ -7,19 +7,20 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "tests", "cpp", "wrapper_demo.cpp")
-EXE = os.path.join(ROOT, "tests", "cpp", "wrapper_demo")
+CPP = os.path.join(ROOT, "tests", "cpp")
 
 
-def build_demo():
-    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), SRC,
-                    "-L" + os.path.join(ROOT, "nextsimdg_amd"), "-lnsdg_hip",
-                    "-Wl,-rpath," + os.path.join(ROOT, "nextsimdg_amd"), "-o", EXE], check=True)
-    return EXE
+def build_demo(name="wrapper_demo"):
+    exe = os.path.join(CPP, name)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(CPP, name + ".cpp"), "-L" + os.path.join(ROOT, "nextsimdg_amd"), "-lnsdg_hip",
+                    "-Wl,-rpath," + os.path.join(ROOT, "nextsimdg_amd"), "-o", exe], check=True)
+    return exe
 
 
-def test_cpp_wrapper_compiles_and_links():
-    assert os.path.exists(build_demo())
+@pytest.mark.parametrize("name", ["wrapper_demo", "module_demo"])
+def test_cpp_wrapper_compiles_and_links(name):
+    assert os.path.exists(build_demo(name))
 
 
 @pytest.mark.gpu
@@ -57,6 +58,47 @@ def test_cpp_wrapper_run_matches_oracle(rheology):
     ref = [o.get_field("u", 1).sum(), o.get_field("v", 1).sum(), h.sum(), a.sum()]
     if rheology == "bbm":
         ref.append(o.get_field("damage", 1).sum())
+    assert len(got) == len(ref)
+    for g, want in zip(got, ref):
+        assert g == pytest.approx(want, rel=1e-9, abs=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rheology", ["mevp", "bbm"])
+def test_cpp_module_run_matches_oracle(rheology):
+    """nsdg_module.hpp: MEVPDynamics / BBMDynamics with the benchmark forcing modules."""
+    from oracle import MEB_DEFAULTS, Oracle, OracleDyn
+    from nextsimdg_amd import inputs
+
+    print(f"C++ module run, {rheology}", flush=True)
+    out = subprocess.run([build_demo("module_demo"), "3", rheology], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = list(map(float, out.stdout.split()))
+    nx = ny = 24
+    bbm = rheology == "bbm"
+    coords, mask = inputs.box_coords(nx, ny), inputs.box_mask(nx, ny)
+    x, y = inputs.cell_corners(nx, ny)
+    h = 0.3 + 0.005 * (np.sin(60e-3 * x / 1e3) + np.sin(30e-3 * y / 1e3))
+    a = np.ones(nx * ny)
+    o = OracleDyn(2, 6, bbm=bbm, meb=MEB_DEFAULTS if bbm else None)
+    o.initialise(nx, ny, coords, mask)
+    zero = np.zeros(nx * ny)
+    d = np.where(mask == 1.0, 1.0, 1.7e38)
+    for name, v in (("hice", h), ("cice", a), ("u", zero), ("v", zero)):
+        o.set_field(name, v, 1)
+    uo, vo = inputs.ocean_gyre(nx, ny)
+    ora = Oracle()
+    for s in range(3):
+        uw, vw = ora.benchmark_wind(nx, ny, inputs.L_BOX, s * 120.0, 0.0, 0.0, 30.0)
+        for name, v in (("hice", h), ("cice", a), ("uwind", uw), ("vwind", vw), ("uocean", uo), ("vocean", vo),
+                        ("ssh", zero)) + ((("damage", d),) if bbm else ()):
+            o.set_field(name, v, 1)
+        o.update(120.0)
+        h, a = o.get_field("hice", 1), o.get_field("cice", 1)
+        if bbm:
+            d = o.get_field("damage", 1)
+    ref = [h.sum(), a.sum(), o.get_field("u", 1).sum(), o.get_field("v", 1).sum(),
+           o.get_field("uiostress", 1).sum(), o.get_field("viostress", 1).sum()] + ([d.sum()] if bbm else [])
     assert len(got) == len(ref)
     for g, want in zip(got, ref):
         assert g == pytest.approx(want, rel=1e-9, abs=1e-12)
