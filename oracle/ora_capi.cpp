@@ -175,7 +175,7 @@ struct DynBase {
     virtual ~DynBase() = default;
     virtual Params& params() = 0;
     virtual Mesh& mesh() = 0;
-    virtual void initialise(size_t, size_t, const double*, const double*) = 0;
+    virtual void initialise(size_t, size_t, const double*, const double*, bool) = 0;
     virtual void finishInit() = 0;
     virtual void setData(const std::string&, const double*, int) = 0;
     virtual void getData(const std::string&, double*, int) = 0;
@@ -196,7 +196,10 @@ template <int CG, int DG> struct DynT : DynBase {
     Mesh& mesh() override { return k.mesh; }
     void setRheology(int r) override { k.rheology = r; }
     MEBParams& meb() override { return k.meb; }
-    void initialise(size_t nx, size_t ny, const double* c, const double* m) override { k.initialise(nx, ny, c, m); }
+    void initialise(size_t nx, size_t ny, const double* c, const double* m, bool sph) override
+    {
+        k.initialise(nx, ny, c, m, sph);
+    }
     void finishInit() override { k.finishInit(); }
     void setData(const std::string& n, const double* d, int c) override { k.setData(n, d, c); }
     void getData(const std::string& n, double* o, int c) override { k.getData(n, o, c); }
@@ -555,9 +558,11 @@ void* ora_dyn_new_bbm(int cg, int dg, const double* p, const double* meb)
     return d;
 }
 void ora_dyn_free(void* d) { delete static_cast<DynBase*>(d); }
-int ora_dyn_initialise(void* d, size_t nx, size_t ny, const double* coords, const double* mask)
+// spherical != 0: coords are (longitude, latitude) in radians; the mesh is rotated
+// (DynamicsKernel.hpp:47-51)
+int ora_dyn_initialise(void* d, size_t nx, size_t ny, const double* coords, const double* mask, int spherical)
 {
-    return guard([&] { static_cast<DynBase*>(d)->initialise(nx, ny, coords, mask); });
+    return guard([&] { static_cast<DynBase*>(d)->initialise(nx, ny, coords, mask, spherical != 0); });
 }
 // Initialise from an explicit mesh (landmask, Dirichlet and periodic lists as given).
 int ora_dyn_initialise_mesh(void* d, void* mesh)

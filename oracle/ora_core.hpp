@@ -179,6 +179,30 @@ template <int D> inline void massMatrix(const Mesh& m, size_t eid, double M[D][D
             M[i][j] = s;
         }
 }
+// NextsimDynamics.hpp:34
+constexpr double EarthRadius = 6371000.0;
+// SphericalTools::massMatrix<D> (ParametricTools.hpp:189-202): the mass matrix with the
+// metric weight cos(latitude) at the Gauss points
+template <int D> inline void sphericalMassMatrix(const Mesh& m, size_t eid, double M[D][D])
+{
+    constexpr int G = gp1d(D);
+    double J[G * G], gx[G * G], gy[G * G];
+    jacobian<G>(m, eid, J);
+    gaussPoints<G>(m, eid, gx, gy);
+    const auto& P = PsiT<D, G>::get();
+    const auto& W = WT<G>::get();
+    double A[D][G * G];
+    for (int i = 0; i < D; ++i)
+        for (int q = 0; q < G * G; ++q)
+            A[i][q] = P.v[i][q] * (W.w[q] * J[q] * std::cos(gy[q]));
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            double s = 0;
+            for (int q = 0; q < G * G; ++q)
+                s += A[i][q] * P.v[j][q];
+            M[i][j] = s;
+        }
+}
 // Dense inverse by Gauss-Jordan elimination with partial pivoting (the reference uses
 // Eigen's .inverse(); results agree to rounding).
 template <int N> inline void invert(const double A[N][N], double R[N][N])
@@ -306,13 +330,18 @@ template <int DG> struct Transport {
                     cellX[eid][i * Q + q] = P.x[i][q] * dyT[1][q] - P.y[i][q] * dxT[1][q];
                     cellY[eid][i * Q + q] = P.y[i][q] * dxT[0][q] - P.x[i][q] * dyT[0][q];
                 }
-            // InitializeInverseDGMassMatrix (ParametricMap.cpp:68-87)
+            // InitializeInverseDGMassMatrix (ParametricMap.cpp:68-87); on the sphere the
+            // cell terms are the Cartesian ones (ParametricMap.cpp:49-62 computes cos(lat)
+            // but does not use it) and the inverse mass carries the metric and 1/R
             double M[DG][DG], Mi[DG][DG];
-            massMatrix<DG>(m, eid, M);
+            if (m.spherical)
+                sphericalMassMatrix<DG>(m, eid, M);
+            else
+                massMatrix<DG>(m, eid, M);
             invert<DG>(M, Mi);
             for (int i = 0; i < DG; ++i)
                 for (int j = 0; j < DG; ++j)
-                    invM[eid][i * DG + j] = Mi[i][j];
+                    invM[eid][i * DG + j] = m.spherical ? Mi[i][j] / EarthRadius : Mi[i][j];
         }
     }
 
@@ -638,7 +667,8 @@ template <int DG> inline void limitMin(double* dg, size_t n, double min)
 // (dynamics/src/Interpolations.cpp)
 using Fn = std::function<double(double, double)>;
 
-// Function2DG (Interpolations.cpp:29-72), Cartesian
+// Function2DG (Interpolations.cpp:29-72); on the sphere both the moments and the mass
+// matrix carry cos(latitude)
 template <int DG> inline void function2DG(const Mesh& m, double* phi, const Fn& f)
 {
     constexpr int G = gp1d(DG), Q = G * G;
@@ -654,12 +684,15 @@ template <int DG> inline void function2DG(const Mesh& m, double* phi, const Fn& 
         for (int q = 0; q < Q; ++q)
             fv[q] = f(gx[q], gy[q]);
         jacobian<G>(m, eid, J);
-        massMatrix<DG>(m, eid, M);
+        if (m.spherical)
+            sphericalMassMatrix<DG>(m, eid, M);
+        else
+            massMatrix<DG>(m, eid, M);
         invert<DG>(M, Mi);
         for (int i = 0; i < DG; ++i) {
             double s = 0;
             for (int q = 0; q < Q; ++q)
-                s += P.v[i][q] * (J[q] * W.w[q]) * fv[q];
+                s += P.v[i][q] * (m.spherical ? J[q] * W.w[q] * std::cos(gy[q]) : J[q] * W.w[q]) * fv[q];
             r[i] = s;
         }
         for (int i = 0; i < DG; ++i) {
@@ -685,8 +718,9 @@ template <int CG> inline void cgLocal(const double* cg, size_t cgi, size_t shift
     }
 }
 
-// CG2DG (Interpolations.cpp:74-122), Cartesian: L2 projection per element; the element
-// mass matrix is rebuilt and inverted on every call, as in the reference.
+// CG2DG (Interpolations.cpp:74-122): L2 projection per element; the element mass matrix
+// is rebuilt and inverted on every call, as in the reference. On the sphere both sides
+// carry cos(latitude) (Interpolations.cpp:99-108).
 template <int CG, int DG> inline void CG2DG(const Mesh& m, double* dg, const double* cg)
 {
     constexpr int G = gp1d(DG), Q = G * G, N = cgdofs(CG);
@@ -698,14 +732,16 @@ template <int CG, int DG> inline void CG2DG(const Mesh& m, double* dg, const dou
     for (size_t dgi = 0; dgi < m.nelements; ++dgi) {
         const size_t iy = dgi / m.nx, ix = dgi % m.nx;
         const size_t cgi = CG * shift * iy + CG * ix;
-        double l[N], J[Q], t[Q], r[DG], M[DG][DG], Mi[DG][DG];
+        double l[N], J[Q], t[Q], r[DG], M[DG][DG], Mi[DG][DG], gx[Q], gy[Q];
         cgLocal<CG>(cg, cgi, shift, l);
         jacobian<G>(m, dgi, J);
+        if (m.spherical)
+            gaussPoints<G>(m, dgi, gx, gy);
         for (int q = 0; q < Q; ++q) {
             double s = 0;
             for (int j = 0; j < N; ++j)
                 s += F.v[j][q] * l[j];
-            t[q] = J[q] * W.w[q] * s;
+            t[q] = m.spherical ? J[q] * W.w[q] * std::cos(gy[q]) * s : J[q] * W.w[q] * s;
         }
         for (int i = 0; i < DG; ++i) {
             double s = 0;
@@ -713,7 +749,10 @@ template <int CG, int DG> inline void CG2DG(const Mesh& m, double* dg, const dou
                 s += P.v[i][q] * t[q];
             r[i] = s;
         }
-        massMatrix<DG>(m, dgi, M);
+        if (m.spherical)
+            sphericalMassMatrix<DG>(m, dgi, M);
+        else
+            massMatrix<DG>(m, dgi, M);
         invert<DG>(M, Mi);
         for (int i = 0; i < DG; ++i) {
             double s = 0;
@@ -886,13 +925,16 @@ inline double elementSize(const Mesh& m, size_t eid)
 
 // ------------------------------------------------------------------ momentum map
 // ParametricMomentumMap<CG,DG> (dynamics/src/include/ParametricMap.hpp:62-131,
-// dynamics/src/ParametricMap.cpp:94-356), Cartesian
+// dynamics/src/ParametricMap.cpp:94-356). On the sphere (x, y) = (longitude, latitude):
+// the lumped masses carry cos(latitude), the longitude derivatives 1/R, the latitude
+// derivatives cos(latitude)/R, and divM / iMM hold the metric terms of the divergence and
+// the strain (ParametricMap.cpp:296-345).
 template <int CG> struct MomentumMap {
     static constexpr int S = cg2dgstress(CG), G = gp1d(S), Q = G * G, N = cgdofs(CG);
     const Mesh& m;
     std::vector<double> lumped, lumped1;
-    std::vector<std::array<double, N * S>> divS1, divS2;     // [j][s]
-    std::vector<std::array<double, S * N>> iMgradX, iMgradY; // [s][j]
+    std::vector<std::array<double, N * S>> divS1, divS2, divM; // [j][s]
+    std::vector<std::array<double, S * N>> iMgradX, iMgradY, iMM; // [s][j]
     std::vector<std::array<double, S * Q>> iMJwPSI;          // [s][q]
     std::vector<std::array<double, 16>> dX_SSH, dY_SSH;      // [j][k]
 
@@ -916,10 +958,12 @@ template <int CG> struct MomentumMap {
             for (size_t iy = p; iy < ny; iy += 2)
                 for (size_t ix = 0; ix < nx; ++ix) {
                     const size_t eid = nx * iy + ix;
-                    double J[GG * GG], Me[N];
+                    double J[GG * GG], Me[N], gx[GG * GG], gy[GG * GG];
                     jacobian<GG>(m, eid, J);
+                    if (m.spherical)
+                        gaussPoints<GG>(m, eid, gx, gy);
                     for (int q = 0; q < GG * GG; ++q)
-                        J[q] *= W.w[q];
+                        J[q] = m.spherical ? J[q] * W.w[q] * std::cos(gy[q]) : J[q] * W.w[q];
                     for (int j = 0; j < N; ++j) {
                         double s = 0;
                         for (int q = 0; q < GG * GG; ++q)
@@ -941,10 +985,12 @@ template <int CG> struct MomentumMap {
             for (size_t iy = p; iy < ny; iy += 2)
                 for (size_t ix = 0; ix < nx; ++ix) {
                     const size_t eid = nx * iy + ix;
-                    double J[4], Me[4];
+                    double J[4], Me[4], gx[4], gy[4];
                     jacobian<2>(m, eid, J);
+                    if (m.spherical)
+                        gaussPoints<2>(m, eid, gx, gy);
                     for (int q = 0; q < 4; ++q)
-                        J[q] *= W2.w[q];
+                        J[q] = m.spherical ? J[q] * W2.w[q] * std::cos(gy[q]) : J[q] * W2.w[q];
                     for (int j = 0; j < 4; ++j) {
                         double s = 0;
                         for (int q = 0; q < 4; ++q)
@@ -970,6 +1016,11 @@ template <int CG> struct MomentumMap {
         iMJwPSI.resize(ne);
         dX_SSH.resize(ne);
         dY_SSH.resize(ne);
+        if (m.spherical) {
+            divM.resize(ne);
+            iMM.resize(ne);
+        }
+        const double iR = 1.0 / EarthRadius;
         const auto& P = PsiT<S, G>::get();
         const auto& F = PhiT<CG, G>::get();
         const auto& F1 = PhiT<1, G>::get();
@@ -984,6 +1035,14 @@ template <int CG> struct MomentumMap {
                     Fy[r][q] *= W.w[q];
                 }
             jacobian<G>(m, eid, J);
+            double cl[Q], sl[Q], gx[Q], gy[Q];
+            if (m.spherical) {
+                gaussPoints<G>(m, eid, gx, gy);
+                for (int q = 0; q < Q; ++q) {
+                    cl[q] = std::cos(gy[q]);
+                    sl[q] = std::sin(gy[q]);
+                }
+            }
             double dxc[N][Q], dyc[N][Q], dx1[4][Q], dy1[4][Q];
             for (int j = 0; j < N; ++j)
                 for (int q = 0; q < Q; ++q) {
@@ -995,28 +1054,36 @@ template <int CG> struct MomentumMap {
                     dx1[j][q] = F1.x[j][q] * Fy[1][q] - F1.y[j][q] * Fx[1][q];
                     dy1[j][q] = F1.y[j][q] * Fx[0][q] - F1.x[j][q] * Fy[0][q];
                 }
+            const bool sph = m.spherical;
             for (int j = 0; j < N; ++j)
                 for (int s = 0; s < S; ++s) {
-                    double a = 0, b = 0;
+                    double a = 0, b = 0, c = 0;
                     for (int q = 0; q < Q; ++q) {
                         a += dxc[j][q] * P.v[s][q];
-                        b += dyc[j][q] * P.v[s][q];
+                        b += (sph ? dyc[j][q] * cl[q] : dyc[j][q]) * P.v[s][q];
+                        if (sph)
+                            c += F.v[j][q] * (J[q] * sl[q] * W.w[q]) * P.v[s][q];
                     }
-                    divS1[eid][j * S + s] = a;
-                    divS2[eid][j * S + s] = b;
+                    divS1[eid][j * S + s] = sph ? a * iR : a;
+                    divS2[eid][j * S + s] = sph ? b * iR : b;
+                    if (sph)
+                        divM[eid][j * S + s] = c * iR;
                 }
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 4; ++k) {
                     double a = 0, b = 0;
                     for (int q = 0; q < Q; ++q) {
                         a += dx1[j][q] * F1.v[k][q];
-                        b += dy1[j][q] * F1.v[k][q];
+                        b += (sph ? dy1[j][q] * cl[q] : dy1[j][q]) * F1.v[k][q];
                     }
-                    dX_SSH[eid][j * 4 + k] = a;
-                    dY_SSH[eid][j * 4 + k] = b;
+                    dX_SSH[eid][j * 4 + k] = sph ? a * iR : a;
+                    dY_SSH[eid][j * 4 + k] = sph ? b * iR : b;
                 }
             double M[S][S], Mi[S][S];
-            massMatrix<S>(m, eid, M);
+            if (sph)
+                sphericalMassMatrix<S>(m, eid, M);
+            else
+                massMatrix<S>(m, eid, M);
             invert<S>(M, Mi);
             for (int s = 0; s < S; ++s) {
                 for (int j = 0; j < N; ++j) {
@@ -1027,6 +1094,12 @@ template <int CG> struct MomentumMap {
                     }
                     iMgradX[eid][s * N + j] = a;
                     iMgradY[eid][s * N + j] = b;
+                    if (sph) {
+                        double c = 0;
+                        for (int t = 0; t < S; ++t)
+                            c += Mi[s][t] * divM[eid][j * S + t];
+                        iMM[eid][s * N + j] = c;
+                    }
                 }
                 for (int q = 0; q < Q; ++q) {
                     double a = 0;
@@ -1072,9 +1145,9 @@ template <int CG, int DG> struct Kernel {
     size_t ncg() const { return (CG * mesh.nx + 1) * (CG * mesh.ny + 1); }
 
     // DynamicsKernel::initialise + CGDynamicsKernel::initialise, Cartesian
-    void initialise(size_t nx, size_t ny, const double* coords, const double* mask)
+    void initialise(size_t nx, size_t ny, const double* coords, const double* mask, bool spherical = false)
     {
-        mesh.fromCoordsAndMask(nx, ny, coords, mask);
+        mesh.fromCoordsAndMask(nx, ny, coords, mask, spherical);
         finishInit();
     }
     void finishInit()
@@ -1116,7 +1189,10 @@ template <int CG, int DG> struct Kernel {
             hElem[e] = elementSize(mesh, e);
             double J[Q], M[DG][DG], Mi[DG][DG];
             jacobian<NG>(mesh, e, J);
-            massMatrix<DG>(mesh, e, M);
+            if (mesh.spherical)
+                sphericalMassMatrix<DG>(mesh, e, M);
+            else
+                massMatrix<DG>(mesh, e, M);
             invert<DG>(M, Mi);
             for (int d = 0; d < DG; ++d)
                 for (int q = 0; q < Q; ++q) {
@@ -1416,6 +1492,15 @@ template <int CG, int DG> struct Kernel {
                     e11[dgi * S + s] = a;
                     e22[dgi * S + s] = b;
                     e12[dgi * S + s] = 0.5 * (c + d);
+                    if (mesh.spherical) { // CGDynamicsKernel.cpp:331-334
+                        double mv = 0, mu = 0;
+                        for (int j = 0; j < N; ++j) {
+                            mv += pmap->iMM[dgi][s * N + j] * lv[j];
+                            mu += pmap->iMM[dgi][s * N + j] * lu[j];
+                        }
+                        e11[dgi * S + s] -= mv;
+                        e12[dgi * S + s] += 0.5 * mu;
+                    }
                 }
             }
         }
@@ -1683,8 +1768,18 @@ template <int CG, int DG> struct Kernel {
                                 x += A[j * S + s] * s12[c * S + s];
                                 y += B[j * S + s] * s22[c * S + s];
                             }
-                            dSx[cg_i + cc + r * row] -= a + b;
-                            dSy[cg_i + cc + r * row] -= x + y;
+                            double tx = a + b, ty = x + y;
+                            if (mesh.spherical) { // CGDynamicsKernel.cpp:348-351
+                                double m12 = 0, m11 = 0;
+                                for (int s = 0; s < S; ++s) {
+                                    m12 += pmap->divM[c][j * S + s] * s12[c * S + s];
+                                    m11 += pmap->divM[c][j * S + s] * s11[c * S + s];
+                                }
+                                tx += m12;
+                                ty -= m11;
+                            }
+                            dSx[cg_i + cc + r * row] -= tx;
+                            dSy[cg_i + cc + r * row] -= ty;
                         }
                 }
             }

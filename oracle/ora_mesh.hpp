@@ -8,6 +8,7 @@
 // periodic[seg][k] = {type (0 X-edge, 1 Y-edge), element left/bottom, element right/top, edge id}.
 #pragma once
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <cstddef>
 #include <cstdint>
@@ -20,6 +21,7 @@ namespace ora {
 
 struct Mesh {
     size_t nx = 0, ny = 0, nelements = 0, nnodes = 0;
+    bool spherical = false; // CoordinateSystem == SPHERICAL: vertices are (longitude, latitude) in radians
     std::vector<double> vx, vy;
     std::vector<unsigned char> landmask;
     std::array<std::vector<size_t>, 4> dirichlet;
@@ -31,6 +33,7 @@ struct Mesh {
         ny = ny_;
         nelements = nx * ny;
         nnodes = (nx + 1) * (ny + 1);
+        spherical = false;
         vx.assign(nnodes, 0.0);
         vy.assign(nnodes, 0.0);
         landmask.assign(nelements, 1);
@@ -39,6 +42,21 @@ struct Mesh {
         periodic.clear();
     }
 
+    // correctlongitude (ParametricMesh.hpp:196-210): when a longitude of the element lies
+    // more than 2 pi / 3 from the first one, the negative longitudes are moved up by 2 pi.
+    template <int N> void correctLongitude(double c[N][2]) const
+    {
+        bool problem = false;
+        for (int i = 1; i < N; ++i)
+            if (std::fabs(c[0][0] - c[i][0]) > 2.0 / 3.0 * M_PI) {
+                problem = true;
+                break;
+            }
+        if (problem)
+            for (int i = 0; i < N; ++i)
+                if (c[i][0] < 0)
+                    c[i][0] += 2.0 * M_PI;
+    }
     // coordinatesOfElement (ParametricMesh.hpp:218-232): LL, LR, UL, UR
     void coordinatesOfElement(size_t eid, double c[4][2]) const
     {
@@ -48,12 +66,41 @@ struct Mesh {
             c[k][0] = vx[ids[k]];
             c[k][1] = vy[ids[k]];
         }
+        if (spherical)
+            correctLongitude<4>(c);
     }
-    // edgevector (ParametricMesh.hpp:381-397), Cartesian
+    // edgevector (ParametricMesh.hpp:381-397): on the sphere the longitude difference is
+    // brought back across the +-pi jump
     void edgevector(size_t n1, size_t n2, double t[2]) const
     {
         t[0] = vx[n2] - vx[n1];
         t[1] = vy[n2] - vy[n1];
+        if (spherical) {
+            if (t[0] > 0.5 * M_PI)
+                t[0] -= 2.0 * M_PI;
+            if (t[0] < -0.5 * M_PI)
+                t[0] += 2.0 * M_PI;
+        }
+    }
+    // RotatePoleToGreenland (ParametricMesh.hpp:137-157): rotation by 40 degrees about the
+    // polar axis, then by 15 degrees about the new y axis
+    void rotatePoleToGreenland()
+    {
+        for (size_t i = 0; i < nnodes; ++i) {
+            const double x = std::cos(vy[i]) * std::cos(vx[i]);
+            const double y = std::cos(vy[i]) * std::sin(vx[i]);
+            const double z = std::sin(vy[i]);
+            const double aw = 40.0 * M_PI / 180.0;
+            const double x1 = std::cos(aw) * x - std::sin(aw) * y;
+            const double y1 = std::sin(aw) * x + std::cos(aw) * y;
+            const double z1 = z;
+            const double bw = 15.0 * M_PI / 180.0;
+            const double x2 = std::cos(bw) * x1 - std::sin(bw) * z1;
+            const double y2 = y1;
+            const double z2 = std::sin(bw) * x1 + std::cos(bw) * z1;
+            vy[i] = std::asin(z2);
+            vx[i] = std::atan2(y2, x2);
+        }
     }
 
     // coordinatesFromModelArray (ParametricMesh.cpp:198-210): coords is N_v x 2 row-major
@@ -103,10 +150,14 @@ struct Mesh {
                 dirichlet[edge].push_back(idx);
         sortDirichlet(edge);
     }
-    // The sequence DynamicsKernel::initialise runs (DynamicsKernel.hpp:44-57), Cartesian only.
-    void fromCoordsAndMask(size_t nx_, size_t ny_, const double* coords, const double* mask)
+    // The sequence DynamicsKernel::initialise runs (DynamicsKernel.hpp:44-57); spherical
+    // coordinates arrive in radians and are rotated.
+    void fromCoordsAndMask(size_t nx_, size_t ny_, const double* coords, const double* mask, bool sph = false)
     {
         coordinatesFromArray(nx_, ny_, coords);
+        spherical = sph;
+        if (sph)
+            rotatePoleToGreenland();
         landmaskFromArray(mask);
         dirichletFromMask();
         for (int e = 0; e < 4; ++e)

@@ -70,7 +70,7 @@ def load():
         "ora_dyn_new": (_vp, [ctypes.c_int, ctypes.c_int, _dp]),
         "ora_dyn_new_bbm": (_vp, [ctypes.c_int, ctypes.c_int, _dp, _dp]),
         "ora_dyn_free": (None, [_vp]),
-        "ora_dyn_initialise": (ctypes.c_int, [_vp, _sz, _sz, _dp, _dp]),
+        "ora_dyn_initialise": (ctypes.c_int, [_vp, _sz, _sz, _dp, _dp, ctypes.c_int]),
         "ora_dyn_initialise_mesh": (ctypes.c_int, [_vp, _vp]),
         "ora_dyn_set_field": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.c_int]),
         "ora_dyn_get_field": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.c_int]),
@@ -261,9 +261,11 @@ class OracleDyn:
             self.lib.ora_dyn_free(self.h)
             self.h = None
 
-    def initialise(self, nx, ny, coords, mask):
+    def initialise(self, nx, ny, coords, mask, spherical=False):
+        """coords: vertices (x, y), or (longitude, latitude) in radians when spherical (the
+        mesh is then rotated, DynamicsKernel.hpp:47-51)."""
         c, m = _f64(coords), _f64(mask)
-        _check(self.lib.ora_dyn_initialise(self.h, nx, ny, _p(c), _p(m)))
+        _check(self.lib.ora_dyn_initialise(self.h, nx, ny, _p(c), _p(m), int(spherical)))
         self.nx, self.ny = nx, ny
 
     def initialise_mesh(self, mesh):
