@@ -70,7 +70,9 @@ int nsdg_device_count(void);
 
 /* ---------------------------------------------------------------- ParametricMesh
  * coordinatesFromModelArray (ParametricMesh.cpp:198-210): coords is (nx+1)(ny+1) x 2.
- * spherical != 0 is rejected (NSDG_ERR_UNSUPPORTED): Cartesian meshes only. */
+ * spherical != 0: (longitude, latitude) in radians, rotated by RotatePoleToGreenland
+ * (ParametricMesh.hpp:137-157) as DynamicsKernel::initialise does (DynamicsKernel.hpp:47-51);
+ * the operators then carry the metric (SphericalTools, ParametricMap.cpp:49-345). */
 int nsdg_mesh_create(size_t nx, size_t ny, const double* coords, int spherical, nsdg_mesh** out);
 /* landmaskFromModelArray (ParametricMesh.cpp:217-223): element is ocean iff mask == 1. */
 int nsdg_mesh_set_landmask(nsdg_mesh* m, const double* mask);
@@ -133,7 +135,9 @@ int nsdg_meb_params_default(nsdg_meb_params* p);
 int nsdg_bbm_create(int cg_degree, int dg_adv, const nsdg_params* params, const nsdg_meb_params* meb, nsdg_dyn** out);
 /* initialise(coords, isSpherical, mask) (DynamicsKernel.hpp:44-75,
  * CGDynamicsKernel.cpp:23-51). Builds the mesh, its Dirichlet lists, the transport and
- * momentum operators on the device. Stress and strain start at zero. */
+ * momentum operators on the device. Stress and strain start at zero. With spherical != 0
+ * the coordinates are (longitude, latitude) in radians (the module converts degrees,
+ * MEVPDynamics.cpp:46-48); such meshes run the per-element operator path (FP64, mEVP). */
 int nsdg_dyn_initialise(nsdg_dyn* d, size_t nx, size_t ny, const double* coords, int spherical,
     const double* mask);
 /* Initialise from an explicitly built mesh (landmask, Dirichlet and periodic lists as set). */

@@ -98,9 +98,8 @@ int nsdg_mesh_create(size_t nx, size_t ny, const double* coords, int spherical, 
 {
     return guard([&] {
         nonNull(out, "out");
-        require(!spherical, "spherical coordinates are not supported", NSDG_ERR_UNSUPPORTED);
         auto m = std::make_unique<nsdg_mesh>();
-        m->m.create(nx, ny, coords);
+        m->m.create(nx, ny, coords, spherical != 0);
         *out = m.release();
     });
 }
@@ -328,9 +327,8 @@ int nsdg_dyn_initialise(nsdg_dyn* d, size_t nx, size_t ny, const double* coords,
 {
     return guard([&] {
         nonNull(d, "dyn");
-        require(!spherical, "spherical coordinates are not supported", NSDG_ERR_UNSUPPORTED);
         HostMesh m;
-        m.create(nx, ny, coords);
+        m.create(nx, ny, coords, spherical != 0);
         m.setLandmask(mask);
         m.dirichletFromMask();
         for (int e = 0; e < 4; ++e)

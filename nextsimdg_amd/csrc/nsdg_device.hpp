@@ -23,7 +23,10 @@ template <bool UNI> struct Ops {
 };
 
 // ---------------------------------------------------------------- geometry
-// Vertices of element e: LL, LR, UL, UR (ParametricMesh.hpp:218-232).
+// Vertices of element e: LL, LR, UL, UR (ParametricMesh.hpp:218-232). On the sphere
+// the longitudes are made continuous across the +-pi jump: when one lies more than
+// 2 pi / 3 from the first, the negative ones move up by 2 pi (correctlongitude,
+// ParametricMesh.hpp:196-210).
 __device__ __forceinline__ void elementCoords(const MeshView& m, int e, double c[4][2])
 {
     const int ix = e % m.nx, iy = e / m.nx;
@@ -34,6 +37,29 @@ __device__ __forceinline__ void elementCoords(const MeshView& m, int e, double c
         c[k][0] = m.vxy[2 * ids[k]];
         c[k][1] = m.vxy[2 * ids[k] + 1];
     }
+    if (m.spherical) {
+        bool jump = false;
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            jump = jump || fabs(c[0][0] - c[k][0]) > 2.0 / 3.0 * M_PI;
+        if (jump)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (c[k][0] < 0)
+                    c[k][0] += 2.0 * M_PI;
+    }
+}
+// Longitude difference of an edge on the sphere brought back across the +-pi jump
+// (edgevector, ParametricMesh.hpp:381-397).
+__device__ __forceinline__ double edgeDx(const MeshView& m, double d)
+{
+    if (m.spherical) {
+        if (d > 0.5 * M_PI)
+            d -= 2.0 * M_PI;
+        if (d < -0.5 * M_PI)
+            d += 2.0 * M_PI;
+    }
+    return d;
 }
 // Gradient of the bilinear map at the G x G Gauss points: dxT[r][q] = d x_r / d xi,
 // dyT[r][q] = d x_r / d eta (ParametricTools.hpp:73-86).
@@ -72,6 +98,51 @@ template <int D> __device__ void massMatrix(const double c[4][2], double M[D][D]
             double s = 0;
             for (int q = 0; q < Q; ++q)
                 s += Psi<D, G>::v[i][q] * (W<G>::v[0][q] * J[q]) * Psi<D, G>::v[j][q];
+            M[i][j] = s;
+        }
+}
+// Latitude at the G x G Gauss points (getGaussPointsInElement, ParametricTools.hpp:159-164).
+template <int G> __device__ void gaussLatitude(const double c[4][2], double lat[G * G])
+{
+#pragma unroll
+    for (int q = 0; q < G * G; ++q) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            s += c[k][1] * Phi<1, G>::v[k][q];
+        lat[q] = s;
+    }
+}
+// Metric weight of the element integrals at the G x G Gauss points: cos(latitude) on the
+// sphere, 1 otherwise.
+template <int G> __device__ void metricWeight(const MeshView& m, const double c[4][2], double w[G * G])
+{
+    if (m.spherical) {
+        gaussLatitude<G>(c, w);
+        for (int q = 0; q < G * G; ++q)
+            w[q] = cos(w[q]);
+    } else {
+        for (int q = 0; q < G * G; ++q)
+            w[q] = 1.0;
+    }
+}
+// The element mass matrix with the metric weight: SphericalTools::massMatrix
+// (ParametricTools.hpp:189-202) on the sphere, the Cartesian one otherwise.
+template <int D> __device__ void metricMassMatrix(const MeshView& m, const double c[4][2], double M[D][D])
+{
+    if (!m.spherical) {
+        massMatrix<D>(c, M);
+        return;
+    }
+    constexpr int G = gp1d(D), Q = G * G;
+    double J[Q], cl[Q];
+    jacobian<G>(c, J);
+    metricWeight<G>(m, c, cl);
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            double s = 0;
+            for (int q = 0; q < Q; ++q)
+                s += Psi<D, G>::v[i][q] * (W<G>::v[0][q] * J[q] * cl[q]) * Psi<D, G>::v[j][q];
             M[i][j] = s;
         }
 }

@@ -30,14 +30,19 @@ namespace nsdg {
 
 // ---------------------------------------------------------------- momentum operators
 // Layout (k-major): iMgradX[s*N+j], iMgradY, iMJwPSI[s*Q+q], divS1[j*S+s], divS2,
-// dX_SSH[j*4+k], dY_SSH (ParametricMap.hpp:62-131).
+// dX_SSH[j*4+k], dY_SSH (ParametricMap.hpp:62-131); on spherical meshes K + 2 S N values
+// with the metric terms divM[j*S+s] and iMM[s*N+j] appended.
 template <int CG> struct MomLayout {
     static constexpr int S = cg2dgstress(CG), G = gp1d(S), Q = G * G, N = cgdofs(CG);
     static constexpr int GX = 0, GY = S * N, B = 2 * S * N, D1 = 2 * S * N + S * Q, D2 = 3 * S * N + S * Q,
-                         SX = 4 * S * N + S * Q, SY = SX + 16, QX = SY + 16, QY = QX + Q * N, K = QY + Q * N;
+                         SX = 4 * S * N + S * Q, SY = SX + 16, K = SY + 16, DM = K, IMM = K + N * S,
+                         KS = K + 2 * N * S;
 };
 
-// InitializeDivSMatrices (ParametricMap.cpp:209-297), Cartesian.
+// InitializeDivSMatrices (ParametricMap.cpp:209-356). On the sphere (x, y) = (longitude,
+// latitude): longitude derivatives carry 1 / R, latitude derivatives cos(latitude) / R,
+// the mass matrix cos(latitude), and divM = (PHI_j, PSI_s J sin(latitude) w) / R with
+// iMM = M^-1 divM^T hold the metric terms of the divergence and the strain.
 template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* ops)
 {
     using L = MomLayout<CG>;
@@ -46,28 +51,46 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
     if (e >= nOps)
         return;
     auto put = [&](int k, double v) { ops[static_cast<size_t>(k) * nOps + e] = v; };
-    double c[4][2], Fx[2][Q], Fy[2][Q], J[Q];
+    const bool sph = m.spherical != 0;
+    const double iR = 1.0 / kEarthRadius;
+    double c[4][2], Fx[2][Q], Fy[2][Q], J[Q], cl[Q], sl[Q];
     elementCoords(m, e, c);
     mapGradients<G>(c, Fx, Fy);
     jacobian<G>(c, J);
+    if (sph) {
+        gaussLatitude<G>(c, sl);
+        for (int q = 0; q < Q; ++q) {
+            cl[q] = cos(sl[q]);
+            sl[q] = sin(sl[q]);
+        }
+    }
     for (int r = 0; r < 2; ++r)
         for (int q = 0; q < Q; ++q) {
             Fx[r][q] *= W<G>::v[0][q];
             Fy[r][q] *= W<G>::v[0][q];
         }
     // mapped, weighted CG gradients: dx_cg = PHIx Fy1 - PHIy Fx1, dy_cg = PHIy Fx0 - PHIx Fy0
-    double d1[N][S], d2[N][S];
+    double d1[N][S], d2[N][S], dm[N][S];
     for (int j = 0; j < N; ++j)
         for (int s = 0; s < S; ++s) {
-            double a = 0, b = 0;
+            double a = 0, b = 0, t = 0;
             for (int q = 0; q < Q; ++q) {
                 const double dxc = Phi<CG, G>::x[j][q] * Fy[1][q] - Phi<CG, G>::y[j][q] * Fx[1][q];
                 const double dyc = Phi<CG, G>::y[j][q] * Fx[0][q] - Phi<CG, G>::x[j][q] * Fy[0][q];
                 a += dxc * Psi<S, G>::v[s][q];
-                b += dyc * Psi<S, G>::v[s][q];
+                b += (sph ? dyc * cl[q] : dyc) * Psi<S, G>::v[s][q];
+                if (sph)
+                    t += Phi<CG, G>::v[j][q] * (J[q] * sl[q] * W<G>::v[0][q]) * Psi<S, G>::v[s][q];
+            }
+            if (sph) {
+                a *= iR;
+                b *= iR;
+                t *= iR;
+                put(L::DM + j * S + s, t);
             }
             d1[j][s] = a;
             d2[j][s] = b;
+            dm[j][s] = t;
             put(L::D1 + j * S + s, a);
             put(L::D2 + j * S + s, b);
         }
@@ -78,13 +101,13 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
                 const double dxc = Phi<1, G>::x[j][q] * Fy[1][q] - Phi<1, G>::y[j][q] * Fx[1][q];
                 const double dyc = Phi<1, G>::y[j][q] * Fx[0][q] - Phi<1, G>::x[j][q] * Fy[0][q];
                 a += dxc * Phi<1, G>::v[k][q];
-                b += dyc * Phi<1, G>::v[k][q];
+                b += (sph ? dyc * cl[q] : dyc) * Phi<1, G>::v[k][q];
             }
-            put(L::SX + j * 4 + k, a);
-            put(L::SY + j * 4 + k, b);
+            put(L::SX + j * 4 + k, sph ? a * iR : a);
+            put(L::SY + j * 4 + k, sph ? b * iR : b);
         }
     double M[S][S], Mi[S][S];
-    massMatrix<S>(c, M);
+    metricMassMatrix<S>(m, c, M);
     invert<S>(M, Mi);
     for (int s = 0; s < S; ++s) {
         for (int j = 0; j < N; ++j) {
@@ -95,6 +118,12 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
             }
             put(L::GX + s * N + j, a);
             put(L::GY + s * N + j, b);
+            if (sph) {
+                double t = 0;
+                for (int r = 0; r < S; ++r)
+                    t += Mi[s][r] * dm[j][r];
+                put(L::IMM + s * N + j, t);
+            }
         }
         for (int q = 0; q < Q; ++q) {
             double a = 0;
@@ -103,22 +132,6 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
             put(L::B + s * Q + q, a);
         }
     }
-    // Gauss-point strain straight from the nodal velocities: PSI^T iMgradX/Y (Q x N)
-    for (int q = 0; q < Q; ++q)
-        for (int j = 0; j < N; ++j) {
-            double a = 0, b = 0;
-            for (int s = 0; s < S; ++s) {
-                double gx = 0, gy = 0;
-                for (int t = 0; t < S; ++t) {
-                    gx += Mi[s][t] * d1[j][t];
-                    gy += Mi[s][t] * d2[j][t];
-                }
-                a += Psi<S, G>::v[s][q] * gx;
-                b += Psi<S, G>::v[s][q] * gy;
-            }
-            put(L::QX + q * N + j, a);
-            put(L::QY + q * N + j, b);
-        }
 }
 
 // InitializeLumpedCGMassMatrix (ParametricMap.cpp:94-206): node-centric gather of the
@@ -132,12 +145,13 @@ template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
         return;
     double acc = 0;
     forEachAdjacent<CG, true>(n % row, n / row, m.nx, m.ny, [&](int e, int loc) {
-        double c[4][2], J[GG * GG];
+        double c[4][2], J[GG * GG], cl[GG * GG];
         elementCoords(m, e, c);
         jacobian<GG>(c, J);
+        metricWeight<GG>(m, c, cl); // cos(latitude) on the sphere (ParametricMap.cpp:121-131)
         double me = 0;
         for (int q = 0; q < GG * GG; ++q)
-            me += Phi<CG, GG>::v[loc][q] * (J[q] * W<GG>::v[0][q]);
+            me += Phi<CG, GG>::v[loc][q] * (J[q] * W<GG>::v[0][q] * cl[q]);
         acc += me;
     });
     (void)N;
@@ -321,6 +335,7 @@ struct ElemArgs {
     float* sigF;      // FP32 variant: stress and strength in single precision
     const float* PgF;
     uint32_t iter;    // subcycle counter (random bits of the FP32 stochastic rounding)
+    int sph;          // spherical mesh: the metric terms iMM of the strain (operator tables)
     double Pstar, DM2, invAlpha, relax;
     double xiX, etaX, xiY, etaY; // inverse map gradient of an affine uniform mesh
 };
@@ -375,6 +390,23 @@ __device__ __forceinline__ void strainLocal(const ElemArgs& a, int e, double* e1
         e11[s] = x;
         e22[s] = y;
         e12[s] = 0.5 * (p + q);
+    }
+    if constexpr (!UNI) {
+        // spherical metric terms (CGDynamicsKernel.cpp:331-334): e11 -= iMM v, e12 += iMM u / 2
+        if (a.sph) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                double mv = 0, mu = 0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const double im = op(L::IMM + s * N + j, e);
+                    mv += im * lv[j];
+                    mu += im * lu[j];
+                }
+                e11[s] -= mv;
+                e12[s] += 0.5 * mu;
+            }
+        }
     }
 }
 
@@ -843,6 +875,7 @@ struct NodeArgs {
     double* dSy;
     double rho_ice, F_atm, F_ocean, fc, gravity, beta, dt;
     double xiX, etaX, xiY, etaY, detJ; // affine uniform mesh geometry
+    int sph;                           // spherical mesh: the metric terms divM of the divergence
 };
 
 // updateMomentum for one node (VPCGDynamicsKernel.hpp:132-172), FP64: the node's inputs
@@ -894,6 +927,27 @@ __device__ __forceinline__ void momentumNode(const NodeArgs& a, int n, double dS
 
 // Weak-divergence contributions of one element to its CG nodes:
 // tx_j = (divS1 s11 + divS2 s12)_j, ty_j = (divS1 s12 + divS2 s22)_j (CGDynamicsKernel.cpp:339-363).
+// Spherical metric terms of addStressTensorCell (CGDynamicsKernel.cpp:348-351):
+// tx += divM s12, ty -= divM s11.
+template <int CG, class O>
+__device__ __forceinline__ void divergenceMetric(const NodeArgs& a, const O& op, int e, const double* s11,
+    const double* s12, double* tx, double* ty)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, N = cgdofs(CG);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double m12 = 0, m11 = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double dm = op(L::DM + j * S + s, e);
+            m12 += dm * s12[s];
+            m11 += dm * s11[s];
+        }
+        tx[j] += m12;
+        ty[j] -= m11;
+    }
+}
 template <int CG, bool UNI>
 __device__ __forceinline__ void divergenceLocal(const NodeArgs& a, int e, double* tx, double* ty)
 {
@@ -916,6 +970,9 @@ __device__ __forceinline__ void divergenceLocal(const NodeArgs& a, int e, double
         tx[j] = p1 + p2;
         ty[j] = p3 + p4;
     }
+    if constexpr (!UNI)
+        if (a.sph)
+            divergenceMetric<CG>(a, op, e, s11, s12, tx, ty);
 }
 // The same on a uniform (affine) mesh with literal reference-element values and sum
 // factorisation: stress at the Gauss points, the Jacobian-weighted fluxes
@@ -1253,8 +1310,22 @@ template <int CG, bool UNI> __global__ __launch_bounds__(kBlock) void k_divergen
             p3 += d1 * s12[s];
             p4 += d2 * s22[s];
         }
-        dx -= p1 + p2;
-        dy -= p3 + p4;
+        double tx = p1 + p2, ty = p3 + p4;
+        if constexpr (!UNI) {
+            if (a.sph) { // CGDynamicsKernel.cpp:348-351
+                double m12 = 0, m11 = 0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const double dm = op(L::DM + j * S + s, e);
+                    m12 += dm * s12[s];
+                    m11 += dm * s11[s];
+                }
+                tx += m12;
+                ty -= m11;
+            }
+        }
+        dx -= tx;
+        dy -= ty;
     });
     a.dSx[n] = dx;
     a.dSy[n] = dy;
@@ -2034,7 +2105,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         tr->setScheme(2); // DynamicsKernel.hpp:61
         cg2dg.build(CG, DG, mesh, s);
         const int nOps = mesh.uniform ? 1 : static_cast<int>(ne);
-        mops.alloc(static_cast<size_t>(L::K) * nOps);
+        mops.alloc(static_cast<size_t>(mesh.spherical ? L::KS : L::K) * nOps);
         k_momentum_ops<CG><<<blocksFor(nOps), kBlock, 0, s>>>(view(mesh), nOps, mops.p);
         checkLaunch("momentum_ops");
         lumped.alloc(nn);
@@ -2364,6 +2435,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.sigF = sigF.p;
         a.PgF = strengthF.p;
         a.iter = subcycleCount;
+        a.sph = mesh.spherical ? 1 : 0;
         a.Pstar = p.Pstar;
         a.DM2 = p.DeltaMin * p.DeltaMin;
         a.invAlpha = 1.0 / p.alpha;
@@ -2417,6 +2489,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.xiY = mesh.xiY;
         a.etaY = mesh.etaY;
         a.detJ = mesh.detJ;
+        a.sph = mesh.spherical ? 1 : 0;
         return a;
     }
     // Two-pass subcycle (general meshes, periodic meshes); FP64 only, the FP32 variant always

@@ -83,6 +83,7 @@ template <class T> struct DBuf {
 // Dirichlet element lists and the periodic segments.
 struct HostMesh {
     size_t nx = 0, ny = 0;
+    bool spherical = false;     // coords are (longitude, latitude) in radians, rotated at create
     std::vector<double> coords; // (nx+1)(ny+1) x 2
     std::vector<uint8_t> ocean; // 1 = ocean/ice element
     std::array<std::vector<uint64_t>, 4> dirichlet;
@@ -90,7 +91,8 @@ struct HostMesh {
 
     size_t ne() const { return nx * ny; }
     size_t nv() const { return (nx + 1) * (ny + 1); }
-    void create(size_t nx_, size_t ny_, const double* c);
+    void create(size_t nx_, size_t ny_, const double* c, bool sph = false);
+    void rotatePoleToGreenland();
     void setLandmask(const double* mask);
     void dirichletFromMask();
     void dirichletFromEdge(int edge);
@@ -108,6 +110,7 @@ struct DeviceMesh {
     bool affine = false;      // uniform, and element 0 is a parallelogram (constant Jacobian)
     double xiX = 0, etaX = 0, xiY = 0, etaY = 0, detJ = 0; // inverse map gradient when affine
     bool hasPeriodic = false;
+    bool spherical = false;   // (longitude, latitude) vertices: metric terms, never uniform
     DBuf<double> vxy;         // vertices, interleaved x, y
     DBuf<uint8_t> ocean;
     DBuf<uint32_t> eflags;
@@ -126,11 +129,14 @@ struct MeshView {
     const uint32_t* eflags;
     const double* factX;
     const double* factY;
+    int spherical;
 };
 inline MeshView view(const DeviceMesh& m)
 {
-    return { m.nx, m.ny, m.vxy.p, m.ocean.p, m.eflags.p, m.factX.p, m.factY.p };
+    return { m.nx, m.ny, m.vxy.p, m.ocean.p, m.eflags.p, m.factX.p, m.factY.p, m.spherical ? 1 : 0 };
 }
+// NextsimDynamics.hpp:34
+constexpr double kEarthRadius = 6371000.0;
 
 // ---------------------------------------------------------------- kernel timing
 struct Profiler {

@@ -7,17 +7,36 @@
 
 namespace nsdg {
 
-void HostMesh::create(size_t nx_, size_t ny_, const double* c)
+void HostMesh::create(size_t nx_, size_t ny_, const double* c, bool sph)
 {
     require(nx_ >= 2 && ny_ >= 2, "mesh needs at least 2 x 2 elements");
     require(c != nullptr, "coords is NULL");
     nx = nx_;
     ny = ny_;
+    spherical = sph;
     coords.assign(c, c + 2 * nv());
+    if (sph)
+        rotatePoleToGreenland();
     ocean.assign(ne(), 1);
     for (auto& d : dirichlet)
         d.clear();
     periodic.clear();
+}
+
+// RotatePoleToGreenland (ParametricMesh.hpp:137-157), applied by DynamicsKernel::initialise
+// to spherical meshes (DynamicsKernel.hpp:50-51): 40 degrees about the polar axis, then
+// 15 degrees about the new y axis.
+void HostMesh::rotatePoleToGreenland()
+{
+    const double aw = 40.0 * M_PI / 180.0, bw = 15.0 * M_PI / 180.0;
+    for (size_t i = 0; i < nv(); ++i) {
+        const double lon = coords[2 * i], lat = coords[2 * i + 1];
+        const double x = std::cos(lat) * std::cos(lon), y = std::cos(lat) * std::sin(lon), z = std::sin(lat);
+        const double x1 = std::cos(aw) * x - std::sin(aw) * y, y1 = std::sin(aw) * x + std::cos(aw) * y, z1 = z;
+        const double x2 = std::cos(bw) * x1 - std::sin(bw) * z1, y2 = y1, z2 = std::sin(bw) * x1 + std::cos(bw) * z1;
+        coords[2 * i + 1] = std::asin(z2);
+        coords[2 * i] = std::atan2(y2, x2);
+    }
 }
 
 // landmaskFromModelArray (ParametricMesh.cpp:217-223)
@@ -143,7 +162,11 @@ void DeviceMesh::build(const HostMesh& m, hipStream_t s)
         etaY = xXi / detJ;
     }
     // congruent translates on a structured grid are parallelograms; a degenerate or
-    // inverted element 0 leaves the per-element operator path
+    // inverted element 0 leaves the per-element operator path. On the sphere the metric
+    // varies with latitude, so no two elements share operators.
+    spherical = m.spherical;
+    if (spherical)
+        affine = false;
     uniform = affine;
 
     std::vector<uint32_t> fl(ne, 0);

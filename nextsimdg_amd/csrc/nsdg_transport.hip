@@ -59,12 +59,15 @@ template <int DG> __global__ void k_transport_ops(MeshView m, int nOps, double* 
             ops[static_cast<size_t>(i * Q + q) * nOps + e] = X;
             ops[static_cast<size_t>(DG * Q + i * Q + q) * nOps + e] = Y;
         }
+    // on the sphere: the cell terms stay the Cartesian ones in (longitude, latitude) and the
+    // inverse mass carries cos(latitude) and 1 / R (ParametricMap.cpp:49-80)
     double M[DG][DG], Mi[DG][DG];
-    massMatrix<DG>(c, M);
+    metricMassMatrix<DG>(m, c, M);
     invert<DG>(M, Mi);
     for (int i = 0; i < DG; ++i)
         for (int j = 0; j < DG; ++j)
-            ops[static_cast<size_t>(2 * DG * Q + i * DG + j) * nOps + e] = Mi[i][j];
+            ops[static_cast<size_t>(2 * DG * Q + i * DG + j) * nOps + e]
+                = m.spherical ? Mi[i][j] / kEarthRadius : Mi[i][j];
 }
 
 // ---------------------------------------------------------------- normal velocities
@@ -87,7 +90,7 @@ template <int DG> __global__ void k_normal_velocity(MeshView m, const double* vx
     if (t < nXe) { // X-edge ie = ix + nx*iy (bottom edge of row iy)
         const int ix = t % nx, iy = t / nx;
         const int n0 = iy * (nx + 1) + ix;
-        const double t0 = m.vxy[2 * (n0 + 1)] - m.vxy[2 * n0];
+        const double t0 = edgeDx(m, m.vxy[2 * (n0 + 1)] - m.vxy[2 * n0]);
         const double t1 = m.vxy[2 * (n0 + 1) + 1] - m.vxy[2 * n0 + 1];
         if (iy > 0 && m.ocean[t - nx]) {
             const int c = t - nx;
@@ -122,7 +125,7 @@ template <int DG> __global__ void k_normal_velocity(MeshView m, const double* vx
         const int ie = t - nXe;
         const int ix = ie % (nx + 1), iy = ie / (nx + 1);
         const int n0 = iy * (nx + 1) + ix;
-        const double t0 = m.vxy[2 * (n0 + nx + 1)] - m.vxy[2 * n0];
+        const double t0 = edgeDx(m, m.vxy[2 * (n0 + nx + 1)] - m.vxy[2 * n0]);
         const double t1 = m.vxy[2 * (n0 + nx + 1) + 1] - m.vxy[2 * n0 + 1];
         const int cl = ix - 1 + nx * iy, cr = ix + nx * iy;
         if (ix > 0 && m.ocean[cl]) {
@@ -463,17 +466,19 @@ __global__ void k_limit(double* f, int ne, bool doMax, double maxv, bool doMin, 
 }
 
 // ---------------------------------------------------------------- CG -> DG
-// Composite per-element projection op[i*N + j] = (M^-1 PSI diag(J w) PHI^T)[i][j].
+// Composite per-element projection op[i*N + j] = (M^-1 PSI diag(J w c) PHI^T)[i][j], c the
+// metric weight (cos(latitude) on the sphere, Interpolations.cpp:99-108).
 template <int CG, int DG> __global__ void k_cg2dg_ops(MeshView m, int nOps, double* ops)
 {
     constexpr int G = gp1d(DG), Q = G * G, N = cgdofs(CG);
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nOps)
         return;
-    double c[4][2], J[Q], M[DG][DG], Mi[DG][DG];
+    double c[4][2], J[Q], cl[Q], M[DG][DG], Mi[DG][DG];
     elementCoords(m, e, c);
     jacobian<G>(c, J);
-    massMatrix<DG>(c, M);
+    metricWeight<G>(m, c, cl);
+    metricMassMatrix<DG>(m, c, M);
     invert<DG>(M, Mi);
     for (int i = 0; i < DG; ++i)
         for (int j = 0; j < N; ++j) {
@@ -481,7 +486,7 @@ template <int CG, int DG> __global__ void k_cg2dg_ops(MeshView m, int nOps, doub
             for (int t = 0; t < DG; ++t) {
                 double r = 0;
                 for (int q = 0; q < Q; ++q)
-                    r += Psi<DG, G>::v[t][q] * (J[q] * W<G>::v[0][q]) * Phi<CG, G>::v[j][q];
+                    r += Psi<DG, G>::v[t][q] * (J[q] * W<G>::v[0][q] * cl[q]) * Phi<CG, G>::v[j][q];
                 s += Mi[i][t] * r;
             }
             ops[static_cast<size_t>(i * N + j) * nOps + e] = s;

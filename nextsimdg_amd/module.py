@@ -69,8 +69,8 @@ class IDynamics:
 
     def setData(self, ms):
         """MEVPDynamics::setData (MEVPDynamics.cpp:37-57): initialise the kernel from the
-        coordinates and mask, then set h, A, u, v. Spherical coordinates are converted to
-        radians and passed on (the kernel rejects them: Cartesian meshes only)."""
+        coordinates and mask, then set h, A, u, v. Spherical coordinates (longitude,
+        latitude in degrees) are converted to radians for the kernel."""
         spherical = _check_spherical(ms)
         nx, ny = ms["nx"], ms["ny"]
         coords = np.asarray(ms["coords"], dtype=np.float64)
