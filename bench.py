@@ -86,6 +86,60 @@ def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0, kind="plain"):
     return el, stats
 
 
+def general_byte_model(nx, ny, spherical=True):
+    """Algorithmic HBM bytes of one two-pass subcycle on a mesh with per-element operators
+    (CG2): the element pass reads iMgradX/Y (2 S N), iMJwPSI (S Q) and on the sphere iMM
+    (S N), the stress (read + written), the strength at the Q Gauss points and u, v; the
+    node pass reads divS1/2 (2 N S) and on the sphere divM (N S), the stress, and per node
+    the 9 frozen values, writing u, v."""
+    S, N, Q = 8, 9, 9
+    ne, nn = nx * ny, (2 * nx + 1) * (2 * ny + 1)
+    extra = S * N if spherical else 0
+    elem = 8 * (2 * S * N + S * Q + extra + 6 * S + Q) * ne + 16 * nn
+    node = 8 * (2 * N * S + extra + 3 * S) * ne + 88 * nn
+    return elem, node
+
+
+def spherical_leg(nx, steps, warmup, dt=120.0, seed=0):
+    """The mEVP step on a spherical longitude-latitude patch (per-element operators with
+    the metric terms, streamed from HBM): throughput and the two passes' byte models."""
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    s = inputs.benchmark_setup(nx, nx, seed)
+    k = nd.MEVPDynamicsKernel(2, 6)
+    k.initialise(inputs.sphere_patch_coords(nx, nx), True, s["mask"], nx, nx)
+    for name, key in (("hice", "h"), ("cice", "A"), ("uocean", "uocean"), ("vocean", "vocean")):
+        k.setData(name, s[key], 1)
+    k.setData("ssh", 0 * s["h"], 1)
+    off = inputs.cyclone_offset(seed)
+    run_steps(k, off, inputs.L_BOX, 0, warmup, dt)
+    k.synchronize()
+    t = time.perf_counter()
+    run_steps(k, off, inputs.L_BOX, warmup, steps, dt)
+    k.synchronize()
+    el = time.perf_counter() - t
+    k.set_profiling(True)
+    run_steps(k, off, inputs.L_BOX, warmup + steps, 1, dt)
+    k.synchronize()
+    stats = {n: k.kernel_stats(n) for n in ("mevp_element", "mevp_node", "transport_op", "cg2dg", "dg2cg",
+                                            "frozen_terms")}
+    k.set_profiling(False)
+    eb, nb = general_byte_model(nx, nx)
+    out = {"workload": f"mEVP dynamics step on a spherical longitude-latitude patch {nx}x{nx} (1.2 x 0.4 rad "
+                       f"around 63N), CG2/DG2, FP64, per-element operators with metric terms, 100 subcycles",
+           "value": round(nx * nx * 100 * steps / el / 1e6, 3), "unit": "Mcell-mEVP-iters/s",
+           "steps": steps, "ms_per_step": round(el / steps * 1e3, 4), "kernels": {}}
+    for name, alg in (("mevp_element", eb), ("mevp_node", nb)):
+        c, tot = stats[name]
+        avg = tot / c
+        out["kernels"][name] = {"avg_launch_ms": round(avg, 5), "algorithmic_bytes_per_launch": alg,
+                                "achieved": round(alg / (avg * 1e-3) / 1e9, 1), "unit": "GB/s",
+                                "hbm_frac": round(alg / (avg * 1e-3) / HBM_PEAK, 4)}
+    out["kernels_ms"] = {n: (round(v[1] / v[0], 5) if v[0] else None) for n, v in stats.items()}
+    return out
+
+
 def brittle_byte_model(nx, ny, cg=2, dg=6):
     """Algorithmic HBM bytes of one one-pass brittle substep: stress read + written (3 S
     coefficients each), damage read + written, h and A read (D coefficients each) per
@@ -290,6 +344,7 @@ def main():
     ap.add_argument("--fp32-error-steps", type=int, default=100)
     ap.add_argument("--learned-nx", type=int, default=1024, help="learned vs plain size (0 skips it)")
     ap.add_argument("--bbm-nx", type=int, default=512, help="brittle (BBM) step size (0 skips it)")
+    ap.add_argument("--spherical-nx", type=int, default=512, help="spherical patch size (0 skips it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -363,6 +418,9 @@ def main():
     if args.bbm_nx:
         log(f"bench: BBM brittle step at {args.bbm_nx}^2 ...")
         out["bbm"] = brittle_leg(args.bbm_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
+    if args.spherical_nx:
+        log(f"bench: spherical patch at {args.spherical_nx}^2 ...")
+        out["spherical"] = spherical_leg(args.spherical_nx, max(2, args.steps // 2), 1)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:

@@ -87,3 +87,23 @@ def benchmark_setup(nx, ny, seed=0, kind="plain"):
         h, A = ice_leads(nx, ny, seed)
     uo, vo = ocean_gyre(nx, ny)
     return dict(coords=coords, mask=mask, h=h, A=A, uocean=uo, vocean=vo)
+
+
+def rotate_pole_from_greenland(lon, lat):
+    """RotatePoleFromGreenland (ParametricMesh.hpp:159-180), the inverse of the rotation
+    the kernel applies to spherical meshes: vertices given to the kernel after this
+    rotation end up at (lon, lat) inside it."""
+    x, y, z = np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)
+    aw, bw = np.radians(-40.0), np.radians(-15.0)
+    x1, y1, z1 = np.cos(bw) * x - np.sin(bw) * z, y, np.sin(bw) * x + np.cos(bw) * z
+    x2, y2, z2 = np.cos(aw) * x1 - np.sin(aw) * y1, np.sin(aw) * x1 + np.cos(aw) * y1, z1
+    return np.arctan2(y2, x2), np.arcsin(z2)
+
+
+def sphere_patch_coords(nx, ny, lon0=-0.6, lon1=0.6, lat0=0.9, lat1=1.3):
+    """Vertices (radians) of a longitude-latitude patch, as the kernel sees it after its
+    pole rotation (the default: 1.2 rad x 0.4 rad around 63N, cells of ~5-7 km at 512^2)."""
+    j, i = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    lon = lon0 + (lon1 - lon0) * i.ravel() / nx
+    lat = lat0 + (lat1 - lat0) * j.ravel() / ny
+    return np.stack(rotate_pole_from_greenland(lon, lat), axis=1)
