@@ -354,8 +354,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design "
-              "note in DESIGN.md, not a measured path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path is built and measured on one "
+              "GPU's share of the grid; sharding across GPUs is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
     nx = ny = args.nx
     log(f"bench: {nx}x{ny} CG2/DG2 FP64, warmup {args.warmup}, steps {args.steps}")
