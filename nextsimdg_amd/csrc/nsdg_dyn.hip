@@ -1754,16 +1754,40 @@ __device__ __forceinline__ constexpr double legendre01(int i, double t) { return
 // sum factorisation (ocean elements; land elements interpolate their stored strain), both
 // projections are psi_k(q) w_q / m_k literals, and every DG field is evaluated per Gauss
 // row from its three row coefficients. The strain is not stored.
-template <int CG, int DG, bool POW5>
-__device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const BrittleArgs& b, int e, double* acc1,
-    double* acc2, double* acc3, double* accD)
+// The caller's per-thread column gp(r), r = 4 q + {0, 1, 2, 3} for Gauss point q, holds
+// first the previous stress and damage interpolated there (a pre-pass, one field at a
+// time), then the new ones; the projection runs after the Gauss loop in the order q. So
+// neither the previous coefficients nor the 30 projection sums are live while the point
+// arithmetic runs. Ocean and land elements take separate Gauss loops.
+template <int DG, int NG, class GP>
+__device__ __forceinline__ void bbmInterpolateToGauss(const double* f, size_t ne, int e, int r, GP& gp)
+{
+    double c[DG];
+#pragma unroll
+    for (int i = 0; i < DG; ++i)
+        c[i] = f[i * ne + e];
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy) {
+        const double Y = gauss_point(NG, gy) - 0.5;
+        double rc[3] = { 0, 0, 0 };
+#pragma unroll
+        for (int i = 0; i < DG; ++i)
+            rc[basisFX(i)] += c[i] * legendre01(basisGY(i), Y);
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            const double X = gauss_point(NG, gx) - 0.5, X2 = X * X - 1.0 / 12.0;
+            gp(4 * (gx + NG * gy) + r) = rc[0] + rc[1] * X + rc[2] * X2;
+        }
+    }
+}
+template <int CG, int DG, bool POW5, bool OCEAN, class GP>
+__device__ __forceinline__ void bbmGaussLoop(const ElemArgs& a, const BrittleArgs& b, int e, GP& gp)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
     const size_t ne = a.ne;
-    const bool oc = a.ocean[e] != 0;
-    double buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
-    if (oc) {
+    double buf[OCEAN ? 2 * N : 3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
+    if constexpr (OCEAN) {
         const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
         const int cgi = CG * row * iy + CG * ix;
 #pragma unroll
@@ -1778,42 +1802,24 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
         for (int k = 0; k < 3 * S; ++k)
             buf[k] = a.eps[k * ne + e];
     }
-    double hc[DG], ac[DG], dc[DG], s11[S], s12[S], s22[S];
+    double hc[DG], ac[DG];
 #pragma unroll
     for (int i = 0; i < DG; ++i) {
         hc[i] = a.h[i * ne + e];
         ac[i] = a.A[i * ne + e];
-        dc[i] = b.damageIn[i * ne + e];
     }
-    loadTensor<S>(a.sig, ne, e, s11, s12, s22);
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-        acc1[k] = acc2[k] = acc3[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < DG; ++k)
-        accD[k] = 0.0;
 #pragma unroll
     for (int gy = 0; gy < NG; ++gy) {
         const double Y = gauss_point(NG, gy) - 0.5;
-        // row coefficients of h, A, damage and the three stress components
-        double rh[3] = { 0, 0, 0 }, ra[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 };
-        double r1[3] = { 0, 0, 0 }, r2[3] = { 0, 0, 0 }, r3[3] = { 0, 0, 0 };
+        double rh[3] = { 0, 0, 0 }, ra[3] = { 0, 0, 0 };
 #pragma unroll
         for (int i = 0; i < DG; ++i) {
             const double g = legendre01(basisGY(i), Y);
             rh[basisFX(i)] += hc[i] * g;
             ra[basisFX(i)] += ac[i] * g;
-            rd[basisFX(i)] += dc[i] * g;
-        }
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const double g = legendre01(basisGY(k), Y);
-            r1[basisFX(k)] += s11[k] * g;
-            r2[basisFX(k)] += s12[k] * g;
-            r3[basisFX(k)] += s22[k] * g;
         }
         double Iu[K], Du[K], Iv[K], Dv[K];
-        if (oc) {
+        if constexpr (OCEAN) {
 #pragma unroll
             for (int jx = 0; jx < K; ++jx) {
                 double iu = 0, du = 0, iv = 0, dv = 0;
@@ -1834,10 +1840,8 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
 #pragma unroll
         for (int gx = 0; gx < NG; ++gx) {
             const double X = gauss_point(NG, gx) - 0.5, X2 = X * X - 1.0 / 12.0;
-            double bd[8];
-            dgBasisAt(X, Y, bd);
             double g11 = 0, g12 = 0, g22 = 0;
-            if (oc) {
+            if constexpr (OCEAN) {
                 double uxi = 0, ueta = 0, vxi = 0, veta = 0;
 #pragma unroll
                 for (int jx = 0; jx < K; ++jx) {
@@ -1851,6 +1855,8 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
                 g22 = vxi * a.xiY + veta * a.etaY;
                 g12 = 0.5 * ((uxi * a.xiY + ueta * a.etaY) + (vxi * a.xiX + veta * a.etaX));
             } else {
+                double bd[8];
+                dgBasisAt(X, Y, bd);
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     g11 += buf[k] * bd[k];
@@ -1858,28 +1864,60 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
                     g22 += buf[2 * S + k] * bd[k];
                 }
             }
-            double hg = rh[0] + rh[1] * X + rh[2] * X2, ag = ra[0] + ra[1] * X + ra[2] * X2;
-            double dg = rd[0] + rd[1] * X + rd[2] * X2;
-            double y11 = r1[0] + r1[1] * X + r1[2] * X2, y12 = r2[0] + r2[1] * X + r2[2] * X2;
-            double y22 = r3[0] + r3[1] * X + r3[2] * X2;
+            const int q = gx + NG * gy;
+            const double hg = rh[0] + rh[1] * X + rh[2] * X2, ag = ra[0] + ra[1] * X + ra[2] * X2;
+            double y11 = gp(4 * q + 0), y12 = gp(4 * q + 1), y22 = gp(4 * q + 2), dg = gp(4 * q + 3);
             bbmPoint<POW5>(b, hg, ag, dg, g11, g12, g22, y11, y12, y22);
-            const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy);
+            gp(4 * q + 0) = y11;
+            gp(4 * q + 1) = y12;
+            gp(4 * q + 2) = y22;
+            gp(4 * q + 3) = dg;
+        }
+    }
+}
+template <int CG, int DG, bool POW5, class GP>
+__device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const BrittleArgs& b, int e, double* acc1,
+    double* acc2, double* acc3, double* accD, GP&& gp)
+{
+    using L = MomLayout<CG>;
+    constexpr int S = L::S, NG = L::G;
+    const size_t ne = a.ne;
+    bbmInterpolateToGauss<S, NG>(a.sig + 0 * S * ne, ne, e, 0, gp);
+    bbmInterpolateToGauss<S, NG>(a.sig + 1 * S * ne, ne, e, 1, gp);
+    bbmInterpolateToGauss<S, NG>(a.sig + 2 * S * ne, ne, e, 2, gp);
+    bbmInterpolateToGauss<DG, NG>(b.damageIn, ne, e, 3, gp);
+    if (a.ocean[e])
+        bbmGaussLoop<CG, DG, POW5, true>(a, b, e, gp);
+    else
+        bbmGaussLoop<CG, DG, POW5, false>(a, b, e, gp);
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const double c = bd[k] * w * invMassRef(k);
-                if (c == 0.0) // basis functions vanishing at this Gauss point (a literal test)
-                    continue;
-                acc1[k] += c * y11;
-                acc2[k] += c * y12;
-                acc3[k] += c * y22;
-            }
+    for (int k = 0; k < S; ++k)
+        acc1[k] = acc2[k] = acc3[k] = 0.0;
 #pragma unroll
-            for (int k = 0; k < DG; ++k) {
-                const double c = bd[k] * w * invMassRef(k);
-                if (c == 0.0)
-                    continue;
-                accD[k] += c * dg;
-            }
+    for (int k = 0; k < DG; ++k)
+        accD[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NG * NG; ++q) {
+        const int gx = q % NG, gy = q / NG;
+        double bd[8];
+        dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
+        const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy);
+        const double y11 = gp(4 * q + 0), y12 = gp(4 * q + 1), y22 = gp(4 * q + 2), dg = gp(4 * q + 3);
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double c = bd[k] * w * invMassRef(k);
+            if (c == 0.0) // basis functions vanishing at this Gauss point (a literal test)
+                continue;
+            acc1[k] += c * y11;
+            acc2[k] += c * y12;
+            acc3[k] += c * y22;
+        }
+#pragma unroll
+        for (int k = 0; k < DG; ++k) {
+            const double c = bd[k] * w * invMassRef(k);
+            if (c == 0.0)
+                continue;
+            accD[k] += c * dg;
         }
     }
 }
@@ -1983,12 +2021,16 @@ struct BrittleFusedArgs {
     double* uOut;
     double* vOut;
 };
-template <int CG, int DG, bool POW5> __global__ __launch_bounds__(kBlock) void k_bbm_fused(BrittleFusedArgs f)
+template <int CG, int DG, bool POW5>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_bbm_fused(BrittleFusedArgs f)
 {
     using T = FusedTile<CG>;
-    constexpr int N = T::N, S = cg2dgstress(CG);
-    // divergence contributions, row 2 j + c (node j, component c), column = local element
-    __shared__ double stage[2 * N][kBlock];
+    constexpr int N = T::N, S = cg2dgstress(CG), Q = MomLayout<CG>::Q;
+    // One column per local element. Rows 4 q + c: the Gauss-point stress and damage of the
+    // element update; afterwards rows 2 j + c: its divergence contributions (node j,
+    // component c). Each thread reads and writes only its own column.
+    constexpr int kRows = 4 * Q > 2 * N ? 4 * Q : 2 * N;
+    __shared__ double stage[kRows][kBlock];
     const NodeArgs& a = f.n;
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
     const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
@@ -2006,7 +2048,7 @@ template <int CG, int DG, bool POW5> __global__ __launch_bounds__(kBlock) void k
         if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
             const int e = ex + a.nx * ey;
             double s11[S], s12[S], s22[S], d[DG];
-            bbmElementAffine<CG, DG, POW5>(f.e, f.b, e, s11, s12, s22, d);
+            bbmElementAffine<CG, DG, POW5>(f.e, f.b, e, s11, s12, s22, d, [&](int r) -> double& { return stage[r][le]; });
             if (lx >= 1 && ly >= 1) {
                 storeTensor<S>(f.sigOut, a.ne, e, s11, s12, s22);
 #pragma unroll
