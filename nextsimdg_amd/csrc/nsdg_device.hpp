@@ -289,6 +289,30 @@ __device__ __forceinline__ void forEachAdjacent(int nix, int niy, int nx, int ny
 // contiguous tile range [xcd * chunk, (xcd + 1) * chunk) keeps a tile's left and
 // lower neighbours (its halo) in the same L2. The grid is padded to 8 * chunk blocks;
 // returns -1 for the padding.
+// A value of an SoA field addressed as the field's start (uniform: a scalar register pair)
+// plus the element's byte offset (32 bits, per lane), in the global address space. The
+// start goes through readfirstlane, which the compiler cannot see through, so that it
+// does not fold the per-field offsets into per-load 64-bit vector addresses: all fields
+// of one element then share one offset register (global_load ... vOff, sBase). The start
+// must be the same in every lane (kernel arguments and uniform offsets), and off < 2^32,
+// i.e. fewer than 2^29 elements of 8 bytes (the host checks).
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T& fieldAt(const T* base, uint32_t off)
+{
+    const uint64_t p = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p >> 32));
+    using GP = __attribute__((address_space(1))) char*;
+    GP b = reinterpret_cast<GP>((static_cast<uint64_t>(hi) << 32) | lo);
+    return *reinterpret_cast<__attribute__((address_space(1))) T*>(b + off);
+}
+// The offset, made to depend on x: loads addressed with the result cannot be scheduled
+// before x is computed, which keeps a late group of loads (and its registers) late.
+__device__ __forceinline__ uint32_t after(uint32_t off, double x)
+{
+    asm volatile("" : "+v"(off) : "v"(x));
+    return off;
+}
+template <class T> __device__ __forceinline__ T ldAt(const T* base, uint32_t off) { return fieldAt(base, off); }
 constexpr int kXCD = 8;
 __device__ __forceinline__ int xcdTile(int b, int ntiles)
 {

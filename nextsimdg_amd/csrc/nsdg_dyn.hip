@@ -134,6 +134,93 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
     }
 }
 
+// Per-element geometry of the one-pass subcycle on general meshes, SoA [k][ne]. At each
+// Gauss point q of the stress rule: the weighted map gradients that make J grad(phi),
+//   J dphi/dx = phi_xi y_eta - phi_eta y_xi,  J dphi/dy = phi_eta x_xi - phi_xi x_eta,
+// as a = y_eta w, b = y_xi w, c = x_xi w cos(lat), d = x_eta w cos(lat) (all / R on the
+// sphere), then J w and, on the sphere, J w sin(lat) / R; last the upper triangle of the
+// element's inverse (metric) mass matrix. Every operator of InitializeDivSMatrices
+// (ParametricMap.cpp:209-356) factors through these:
+//   divS1 = sum_q (phi_xi a - phi_eta b) PSI(q),  divS2 = sum_q (phi_eta c - phi_xi d) PSI(q),
+//   divM = sum_q phi JS PSI(q),  iMgradX/Y = M^-1 divS1/2^T,  iMM = M^-1 divM^T,
+//   iMJwPSI = M^-1 PSI diag(J w),
+// so 6 Q + S (S + 1) / 2 values (90 for CG2) replace the 4 S N + 2 S Q tables (432).
+// The geometry pointer as a value the compiler cannot see through: each phase of the
+// element update re-reads the values it needs (L1/L2 hits) instead of keeping every value
+// of the previous phases live in registers.
+__device__ __forceinline__ const double* phaseLocal(const double* p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+template <int CG> struct GeoLayout {
+    static constexpr int S = cg2dgstress(CG), G = gp1d(S), Q = G * G;
+    static constexpr int A = 0, B = Q, C = 2 * Q, D = 3 * Q, JW = 4 * Q, JS = 5 * Q, MI = 6 * Q;
+    static constexpr int K = MI + S * (S + 1) / 2;
+    __host__ __device__ static constexpr int mi(int s, int t)
+    {
+        return s <= t ? MI + s * S - s * (s - 1) / 2 + (t - s) : mi(t, s);
+    }
+};
+// y_c = M x_c (c = 1, 2, 3) for the element's symmetric inverse mass matrix, stored as its
+// upper triangle: each stored entry is loaded once and applied to both of its positions.
+template <int CG>
+__device__ __forceinline__ void inverseMassApply(const double* g, size_t ne, uint32_t off, const double* x1,
+    const double* x2, const double* x3, double* y1, double* y2, double* y3)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        y1[s] = y2[s] = y3[s] = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int t = s; t < S; ++t) {
+            const double m = ldAt(g + static_cast<size_t>(L::mi(s, t)) * ne, off);
+            y1[s] += m * x1[t];
+            y2[s] += m * x2[t];
+            y3[s] += m * x3[t];
+            if (t != s) {
+                y1[t] += m * x1[s];
+                y2[t] += m * x2[s];
+                y3[t] += m * x3[s];
+            }
+        }
+}
+template <int CG> __global__ void k_geometry(MeshView m, int ne, double* geo)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S, G = L::G, Q = L::Q;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    auto put = [&](int k, double v) { geo[static_cast<size_t>(k) * ne + e] = v; };
+    const bool sph = m.spherical != 0;
+    const double iR = sph ? 1.0 / kEarthRadius : 1.0;
+    double c[4][2], Fx[2][Q], Fy[2][Q], J[Q], lat[Q];
+    elementCoords(m, e, c);
+    mapGradients<G>(c, Fx, Fy);
+    jacobian<G>(c, J);
+    if (sph)
+        gaussLatitude<G>(c, lat);
+    for (int q = 0; q < Q; ++q) {
+        const double w = W<G>::v[0][q], cl = sph ? cos(lat[q]) : 1.0;
+        put(L::A + q, Fy[1][q] * w * iR);
+        put(L::B + q, Fx[1][q] * w * iR);
+        put(L::C + q, Fx[0][q] * w * cl * iR);
+        put(L::D + q, Fy[0][q] * w * cl * iR);
+        put(L::JW + q, J[q] * w);
+        put(L::JS + q, sph ? J[q] * sin(lat[q]) * w * iR : 0.0);
+    }
+    double M[S][S], Mi[S][S];
+    metricMassMatrix<S>(m, c, M);
+    invert<S>(M, Mi);
+    for (int s = 0; s < S; ++s)
+        for (int t = s; t < S; ++t)
+            put(L::mi(s, t), Mi[s][t]);
+}
+
 // InitializeLumpedCGMassMatrix (ParametricMap.cpp:94-206): node-centric gather of the
 // element integrals of the CG basis (even element rows first, as the stripes add them).
 template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
@@ -347,6 +434,25 @@ template <int S, class T, class U> __device__ __forceinline__ void loadTensor(co
         a[s] = f[(0 * S + s) * ne + e];
         b[s] = f[(1 * S + s) * ne + e];
         c[s] = f[(2 * S + s) * ne + e];
+    }
+}
+// The same with scalar field bases and the element's 32-bit byte offset (ldAt).
+template <int S, class T> __device__ __forceinline__ void loadTensorAt(const T* f, size_t ne, uint32_t off, T* a, T* b, T* c)
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        a[s] = ldAt(f + (0 * S + s) * ne, off);
+        b[s] = ldAt(f + (1 * S + s) * ne, off);
+        c[s] = ldAt(f + (2 * S + s) * ne, off);
+    }
+}
+template <int S, class T> __device__ __forceinline__ void storeTensorAt(T* f, size_t ne, uint32_t off, const T* a, const T* b, const T* c)
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        fieldAt(f + (0 * S + s) * ne, off) = a[s];
+        fieldAt(f + (1 * S + s) * ne, off) = b[s];
+        fieldAt(f + (2 * S + s) * ne, off) = c[s];
     }
 }
 template <int S, class T> __device__ __forceinline__ void storeTensor(T* f, size_t ne, int e, const T* a, const T* b, const T* c)
@@ -1101,8 +1207,6 @@ template <int CG, int TX_, int TY_> struct TileShape {
     static constexpr int RS = kBlock / TPR;                   // node rows per block sweep
     static constexpr int NB = (CG * TY + RS - 1) / RS;        // nodes per thread per sweep
 };
-// Node pass: 64 x 4 elements, long rows for the 128-byte lines of the stress and node arrays.
-template <int CG> using NodeTile = TileShape<CG, 64, 4>;
 // Fused subcycle: 31 x 7 elements, so that tile and halo are exactly one element per thread.
 template <int CG> using FusedTile = TileShape<CG, 31, 7>;
 static_assert(FusedTile<2>::HX * FusedTile<2>::HY == kBlock, "fused tile must give one element per thread");
@@ -1175,43 +1279,6 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const G& contr
     }
 }
 
-// Fused stressDivergence + updateMomentum + applyBoundaries on a general mesh without
-// periodic segments (per-element operators; uniform meshes run k_mevp_fused instead).
-template <int CG> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs a)
-{
-    using T = NodeTile<CG>;
-    constexpr int N = T::N;
-    __shared__ double contrib[T::HX * T::HY][N][2];
-    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
-    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
-    if (tile < 0)
-        return;
-    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
-
-    // Phase 1: weak-divergence contributions of the tile and its left/bottom halo.
-    for (int le = threadIdx.x; le < T::HX * T::HY; le += kBlock) {
-        const int ex = ex0 - 1 + le % T::HX, ey = ey0 - 1 + le / T::HX;
-        double tx[N], ty[N];
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-            tx[j] = ty[j] = 0.0;
-        if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
-            const int e = ex + a.nx * ey;
-            if (a.ocean[e])
-                divergenceLocal<CG, false>(a, e, tx, ty);
-        }
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            contrib[le][j][0] = tx[j];
-            contrib[le][j][1] = ty[j];
-        }
-    }
-    __syncthreads();
-    // Phase 2
-    nodeTileUpdate<CG, T>(
-        a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, a.u, a.v);
-}
-
 // One whole mEVP subcycle in one pass over a uniform mesh (FP64): per tile, phase 1 runs
 // the element update (strain, stress, projection) of every element of the tile and its
 // left/bottom halo, keeps the new stress in registers, stores it for the tile's own
@@ -1219,7 +1286,8 @@ template <int CG> __global__ __launch_bounds__(kBlock) void k_mevp_node(NodeArgs
 // and turns it into weak-divergence contributions in LDS; phase 2 is the node update.
 // Stress and velocities are ping-ponged: the pass reads sigma^k, u^k, v^k (e.sig, n.u,
 // n.v) and writes sigma^(k+1), u^(k+1), v^(k+1) to other buffers, so no tile reads a
-// neighbour's new values. The arithmetic is that of k_mevp_element then k_mevp_node.
+// neighbour's new values. The arithmetic is that of k_mevp_element, k_divergence,
+// k_momentum and k_boundaries in sequence.
 // R = float: the FP32 variant (stress stored as written by the two-pass FP32 kernels,
 // stochastically rounded; the divergence uses the stored values).
 struct FusedArgs {
@@ -1229,6 +1297,7 @@ struct FusedArgs {
     float* sigFOut;
     double* uOut;
     double* vOut;
+    const double* geo; // GeoLayout tables of a general mesh (k_mevp_fused_geom)
 };
 template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_mevp_fused(FusedArgs f)
 {
@@ -1273,6 +1342,260 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
         for (int j = 0; j < N; ++j) {
             contrib[le][j][0] = tx[j];
             contrib[le][j][1] = ty[j];
+        }
+    }
+    __syncthreads();
+    nodeTileUpdate<CG, T>(
+        a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
+}
+
+// The element update of one mEVP subcycle on a general mesh from its geometry (GeoLayout):
+//  * strain (ocean elements): e = M^-1 sum_q PSI(q) g(q), with g the J-weighted velocity
+//    gradients at q, sum-factorised from the 1-D CG basis, and on the sphere the metric
+//    terms (CGDynamicsKernel.cpp:300-337: e11 = iMgradX u - iMM v, e22 = iMgradY v,
+//    e12 = (iMgradX v + iMgradY u + iMM u) / 2); land elements keep their stored strain;
+//  * stress (MEVPStressUpdateStep.hpp:30-118): sigma_VP at the Gauss points from the
+//    strain evaluated there, then sigma <- (1 - 1/alpha) sigma + M^-1 sum_q PSI(q) J w
+//    sigma_VP(q) / alpha.
+// The same operators as the per-element tables of k_momentum_ops, applied in factored form.
+template <int CG, bool SPH>
+__device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const double* geo, int e, double* s11,
+    double* s12, double* s22)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
+    const size_t ne = a.ne;
+    const uint32_t off = static_cast<uint32_t>(e) * 8u;
+    // The strain of the velocity is computed for every element (the nodes of land elements
+    // exist too) and replaced by the stored one on land: a short branch instead of a long
+    // divergent one around the whole strain computation.
+    double e11[S], e12[S], e22[S];
+    {
+        const double* const g1 = phaseLocal(geo);
+        auto gq = [&](int k) { return ldAt(g1 + static_cast<size_t>(k) * ne, off); };
+        double lu[N], lv[N];
+        {
+            const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+            const uint32_t noff = static_cast<uint32_t>(CG * row * iy + CG * ix) * 8u;
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    lu[c + K * r] = ldAt(a.u + c + r * row, noff);
+                    lv[c + K * r] = ldAt(a.v + c + r * row, noff);
+                }
+        }
+        double t11[S], t12[S], t22[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            t11[s] = t12[s] = t22[s] = 0.0;
+#pragma unroll
+        for (int gy = 0; gy < NG; ++gy) {
+            double Iu[K], Du[K], Iv[K], Dv[K];
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                double iu = 0, du = 0, iv = 0, dv = 0;
+#pragma unroll
+                for (int jy = 0; jy < K; ++jy) {
+                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+                    iu += ph * lu[jx + K * jy];
+                    du += dp * lu[jx + K * jy];
+                    iv += ph * lv[jx + K * jy];
+                    dv += dp * lv[jx + K * jy];
+                }
+                Iu[jx] = iu;
+                Du[jx] = du;
+                Iv[jx] = iv;
+                Dv[jx] = dv;
+            }
+#pragma unroll
+            for (int gx = 0; gx < NG; ++gx) {
+                const int q = gx + NG * gy;
+                double uxi = 0, ueta = 0, vxi = 0, veta = 0, uq = 0, vq = 0;
+#pragma unroll
+                for (int jx = 0; jx < K; ++jx) {
+                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                    uxi += dp * Iu[jx];
+                    ueta += ph * Du[jx];
+                    vxi += dp * Iv[jx];
+                    veta += ph * Dv[jx];
+                    if constexpr (SPH) {
+                        uq += ph * Iu[jx];
+                        vq += ph * Iv[jx];
+                    }
+                }
+                const double ga = gq(L::A + q), gb = gq(L::B + q), gc = gq(L::C + q), gd = gq(L::D + q);
+                double g11 = ga * uxi - gb * ueta;
+                const double g22 = gc * veta - gd * vxi;
+                double g12 = (ga * vxi - gb * veta) + (gc * ueta - gd * uxi);
+                if constexpr (SPH) {
+                    const double js = gq(L::JS + q);
+                    g11 -= js * vq;
+                    g12 += js * uq;
+                }
+                g12 *= 0.5;
+                double bd[8];
+                dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (bd[s] == 0.0) // basis functions vanishing at this Gauss point (a literal test)
+                        continue;
+                    t11[s] += bd[s] * g11;
+                    t12[s] += bd[s] * g12;
+                    t22[s] += bd[s] * g22;
+                }
+            }
+        }
+        const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
+        inverseMassApply<CG>(g1, ne, offM, t11, t12, t22, e11, e12, e22);
+    }
+    if (!a.ocean[e])
+        loadTensorAt<S>(a.eps, ne, off, e11, e12, e22);
+    const double* const g2 = phaseLocal(geo);
+    auto gq = [&](int k) { return ldAt(g2 + static_cast<size_t>(k) * ne, off); };
+    double u11[S], u12[S], u22[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        u11[s] = u12[s] = u22[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NG * NG; ++q) {
+        double bd[8];
+        dgBasisAt(gauss_point(NG, q % NG) - 0.5, gauss_point(NG, q / NG) - 0.5, bd);
+        double g11 = 0, g12 = 0, g22 = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            g11 += e11[s] * bd[s];
+            g12 += e12[s] * bd[s];
+            g22 += e22[s] * bd[s];
+        }
+        const double delta = sqrt(a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12);
+        const double P = ldAt(a.Pg + q * ne, off);
+        const double c = (P * 0.125) / delta; // P/8/Delta
+        const double jw = gq(L::JW + q) * a.invAlpha;
+        const double r11 = jw * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
+        const double r12 = jw * ((2.0 * c) * g12);
+        const double r22 = jw * (c * (5.0 * g22 + 3.0 * g11) - 0.5 * P);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (bd[s] == 0.0)
+                continue;
+            u11[s] += bd[s] * r11;
+            u12[s] += bd[s] * r12;
+            u22[s] += bd[s] * r22;
+        }
+    }
+    // the inverse mass and the previous stress are read only now
+    const uint32_t offM = after(after(after(off, u11[S - 1]), u12[S - 1]), u22[S - 1]);
+    double x11[S], x12[S], x22[S];
+    inverseMassApply<CG>(phaseLocal(geo), ne, offM, u11, u12, u22, x11, x12, x22);
+    loadTensorAt<S>(a.sig, ne, after(offM, x22[S - 1]), s11, s12, s22);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        s11[s] = s11[s] * a.relax + x11[s];
+        s12[s] = s12[s] * a.relax + x12[s];
+        s22[s] = s22[s] * a.relax + x22[s];
+    }
+}
+// Weak-divergence contributions of one element (CGDynamicsKernel.cpp:339-398 with the
+// spherical terms tx += divM s12, ty -= divM s11) from its geometry and its stress
+// coefficients in registers: the stress at the Gauss points, the fluxes
+//   Fx = (a s11 - d s12, c s12 - b s11, JS s12), Fy = (a s12 - d s22, c s22 - b s12, -JS s11)
+// against (phi_xi, phi_eta, phi), sum-factorised per Gauss row.
+template <int CG, bool SPH>
+__device__ __forceinline__ void divergenceGeomFrom(const double* geo, size_t ne, int e, const double* s11,
+    const double* s12, const double* s22, double* tx, double* ty)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
+    const double* const g4 = phaseLocal(geo);
+    const uint32_t off = static_cast<uint32_t>(e) * 8u;
+    auto gq = [&](int k) { return ldAt(g4 + static_cast<size_t>(k) * ne, off); };
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        tx[j] = ty[j] = 0.0;
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy) {
+        double ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
+#pragma unroll
+        for (int jx = 0; jx < K; ++jx)
+            ax[jx] = bx[jx] = ay[jx] = by[jx] = 0.0;
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            const int q = gx + NG * gy;
+            double bd[8];
+            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
+            double v11 = 0, v12 = 0, v22 = 0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                v11 += s11[s] * bd[s];
+                v12 += s12[s] * bd[s];
+                v22 += s22[s] * bd[s];
+            }
+            const double ga = gq(L::A + q), gb = gq(L::B + q), gc = gq(L::C + q), gd = gq(L::D + q);
+            const double fxXi = ga * v11 - gd * v12, fxEta = gc * v12 - gb * v11;
+            const double fyXi = ga * v12 - gd * v22, fyEta = gc * v22 - gb * v12;
+            double fx0 = 0, fy0 = 0;
+            if constexpr (SPH) {
+                const double js = gq(L::JS + q);
+                fx0 = js * v12;
+                fy0 = -js * v11;
+            }
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                ax[jx] += dp * fxXi;
+                bx[jx] += ph * fxEta;
+                ay[jx] += dp * fyXi;
+                by[jx] += ph * fyEta;
+                if constexpr (SPH) {
+                    ax[jx] += ph * fx0;
+                    ay[jx] += ph * fy0;
+                }
+            }
+        }
+#pragma unroll
+        for (int jy = 0; jy < K; ++jy) {
+            const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                tx[jx + K * jy] += ph * ax[jx] + dp * bx[jx];
+                ty[jx + K * jy] += ph * ay[jx] + dp * by[jx];
+            }
+        }
+    }
+}
+// One whole mEVP subcycle in one pass over a general (non-affine or spherical) mesh without
+// periodic segments, FP64: k_mevp_fused's tiles, halo and node phase, with the element
+// update and the divergence computed from the streamed geometry instead of literals.
+template <int CG, int DG, bool SPH> __global__ __launch_bounds__(kBlock) void k_mevp_fused_geom(FusedArgs f)
+{
+    using T = FusedTile<CG>;
+    constexpr int N = T::N, S = cg2dgstress(CG);
+    __shared__ double contrib[T::HX * T::HY][N][2];
+    const NodeArgs& a = f.n;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+    {
+        const int le = threadIdx.x; // one element of tile + halo per thread
+        const int lx = le % T::HX, ly = le / T::HX;
+        const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
+        // Halo and tile elements outside the domain compute a valid neighbour's update
+        // (clamped index) and contribute nothing: no divergent branch around the element.
+        const bool inside = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
+        const int e = min(max(ex, 0), a.nx - 1) + a.nx * min(max(ey, 0), a.ny - 1);
+        double s11[S], s12[S], s22[S], tx[N], ty[N];
+        elementGeomUpdate<CG, SPH>(f.e, f.geo, e, s11, s12, s22);
+        if (inside && lx >= 1 && ly >= 1)
+            storeTensorAt<S>(f.sigOut, a.ne, static_cast<uint32_t>(e) * 8u, s11, s12, s22);
+        divergenceGeomFrom<CG, SPH>(f.geo, a.ne, e, s11, s12, s22, tx, ty);
+        const bool adds = inside && a.ocean[e] != 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            contrib[le][j][0] = adds ? tx[j] : 0.0;
+            contrib[le][j][1] = adds ? ty[j] : 0.0;
         }
     }
     __syncthreads();
@@ -2096,7 +2419,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     int precision = NSDG_PREC_FP64;
     size_t ne = 0, nn = 0, nn1 = 0;
     double deltaT = 0;
-    DBuf<double> mops, lumped, lumped1, h, A, ssh, eps, sig;
+    DBuf<double> mops, geo, lumped, lumped1, h, A, ssh, eps, sig;
     DBuf<double> u, v, u0, v0, uo, vo, ua, va, cgH, cgA, gX, gY, dSx, dSy;
     DBuf<double> tmpDG, tmpDG2, tmpCG, tmpCG2, cg1a, cg1b, cg1c, scratch, dg0a, dg0b, part, res;
     DBuf<uint8_t> dirNode;
@@ -2150,6 +2473,13 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         mops.alloc(static_cast<size_t>(mesh.spherical ? L::KS : L::K) * nOps);
         k_momentum_ops<CG><<<blocksFor(nOps), kBlock, 0, s>>>(view(mesh), nOps, mops.p);
         checkLaunch("momentum_ops");
+        if (!mesh.uniform) {
+            // the one-pass kernel addresses element fields with 32-bit byte offsets
+            require(ne < (size_t(1) << 29), "general meshes need fewer than 2^29 elements", NSDG_ERR_ARG);
+            geo.alloc(static_cast<size_t>(GeoLayout<CG>::K) * ne);
+            k_geometry<CG><<<blocksFor(ne), kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), geo.p);
+            checkLaunch("geometry");
+        }
         lumped.alloc(nn);
         lumped1.alloc(nn1);
         k_lumped_mass<CG, CG == 1 ? 1 : 4><<<blocksFor(nn), kBlock, 0, s>>>(view(mesh), lumped.p);
@@ -2547,19 +2877,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             k_mevp_element<CG, DG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
         checkLaunch("mevp_element");
     }
+    // Node passes of the two-pass subcycle (meshes with periodic segments): the split
+    // sequence in the reference's order, with the periodic averaging of the divergence.
     void nodePass()
     {
-        const NodeArgs a = nodeArgs();
-        if (!mesh.hasPeriodic) {
-            Timed t(prof, s, "mevp_node");
-            using T = NodeTile<CG>;
-            const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-            require(!mesh.uniform, "uniform meshes without periodic segments run the one-pass kernel",
-                NSDG_ERR_STATE);
-            k_mevp_node<CG><<<grid, kBlock, 0, s>>>(a);
-            checkLaunch("mevp_node");
-            return;
-        }
         divergenceAsync();
         momentumAsync();
         boundariesAsync();
@@ -2620,7 +2941,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         }
     }
     // One-pass subcycles: uniform meshes without periodic segments.
-    bool fusedSubcycle() const { return mesh.uniform && !mesh.hasPeriodic; }
+    bool fusedSubcycle() const { return !mesh.hasPeriodic; }
     template <class X> static void swapBuf(DBuf<X>& x, DBuf<X>& y)
     {
         std::swap(x.p, y.p);
@@ -2636,9 +2957,17 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         f.sigFOut = sigFAlt.p;
         f.uOut = uAlt.p;
         f.vOut = vAlt.p;
+        f.geo = geo.p;
         using T = FusedTile<CG>;
         const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-        if (precision == NSDG_PREC_FP32) {
+        if (!mesh.uniform) {
+            require(precision == NSDG_PREC_FP64 && geo.n == static_cast<size_t>(GeoLayout<CG>::K) * ne,
+                "general meshes run the FP64 one-pass subcycle from their geometry tables", NSDG_ERR_STATE);
+            if (mesh.spherical)
+                k_mevp_fused_geom<CG, DG, true><<<grid, kBlock, 0, s>>>(f);
+            else
+                k_mevp_fused_geom<CG, DG, false><<<grid, kBlock, 0, s>>>(f);
+        } else if (precision == NSDG_PREC_FP32) {
             require(sigF.n == 3 * S * ne && sigFAlt.n == sigF.n, "FP32 buffers not allocated", NSDG_ERR_STATE);
             k_mevp_fused<CG, DG, float><<<grid, kBlock, 0, s>>>(f);
         } else {
