@@ -252,6 +252,73 @@ def test_fused_subcycles_equal_split_passes(cg, dg, mesh):
         assert e < tol, (n, e)
 
 
+RAGGED = [(2, 2), (2, 7), (7, 2), (3, 2), (31, 7), (32, 8), (62, 14), (63, 15)]
+
+
+def _mesh_coords(mesh, nx, ny):
+    from nextsimdg_amd import inputs
+
+    if mesh == "box":
+        return box_coords(nx, ny, 2000.0)
+    if mesh == "general":
+        c = box_coords(nx, ny, 2000.0, 1700.0, skew=0.15)
+        c[:, 0] += 120.0 * np.sin(c[:, 1] / 9000.0)
+        return c
+    return inputs.sphere_patch_coords(nx, ny)
+
+
+@pytest.mark.parametrize("mesh", ["box", "general", "sphere"])
+@pytest.mark.parametrize("nx,ny", RAGGED)
+def test_fused_subcycles_on_ragged_meshes(nx, ny, mesh):
+    """Meshes smaller than one tile (down to the smallest the library takes, 2 x 2), exact
+    tile multiples and one element past them: the one-pass kernels (tile halo clamping,
+    the last tile's domain-edge nodes) against the split passes, with a land cell where
+    the mesh has room for one."""
+    cg, dg, S = 2, 6, 8
+    _say(f"ragged {nx}x{ny} {mesh}")
+    nd = _gpu()
+    coords = _mesh_coords(mesh, nx, ny)
+    mask = np.ones(nx * ny)
+    if nx * ny >= 6:
+        mask[(ny // 2) * nx + nx // 2] = 0.0
+    p = nd.default_params()
+    a, b = nd.MEVPDynamicsKernel(cg, dg, params=p), nd.MEVPDynamicsKernel(cg, dg, params=p)
+    ne, ncg = nx * ny, (cg * nx + 1) * (cg * ny + 1)
+    rng = np.random.default_rng(nx * 100 + ny)
+    state = {"hice": np.abs(rng.normal(1.0, 0.3, (ne, dg))) * np.r_[1.0, [0.05] * (dg - 1)],
+             "cice": np.clip(rng.normal(0.9, 0.05, (ne, dg)), 0, 1) * np.r_[1.0, [0.02] * (dg - 1)]}
+    for n in ("s11", "s12", "s22", "e11", "e12", "e22"):
+        state[n] = rng.normal(0, 1e3 if n[0] == "s" else 1e-7, (ne, S))
+    forcing = {n: rng.normal(0, sc, ne) for n, sc in (("uwind", 10), ("vwind", 10), ("uocean", 0.05),
+                                                       ("vocean", 0.05), ("ssh", 0.1))}
+    vel = {n: rng.normal(0, 0.1, ncg) for n in ("u", "v")}
+    for k in (a, b):
+        k.initialise(coords, mesh == "sphere", mask, nx, ny)
+        for n, v in state.items():
+            k.setData(n, v, v.shape[1])
+        for n, v in forcing.items():
+            k.setData(n, v, 1)
+        for n, v in vel.items():
+            k.set_cg(n, v)
+        k.prepareIteration()
+        k.begin_iteration(120.0)
+    a.subcycles(5)
+    for _ in range(5):
+        b.projectVelocityToStrain()
+        b.stressUpdateHighOrder(1)
+        b.stressDivergence()
+        b.updateMomentum()
+        b.applyBoundaries()
+    a.synchronize()
+    tol = 1e-13 if mesh == "box" else 1e-12
+    for n in ("u", "v"):
+        ga, gb = a.get_cg(n), b.get_cg(n)
+        assert np.isfinite(ga).all()
+        assert np.abs(ga - gb).max() <= tol * max(np.abs(gb).max(), 1e-30), n
+    for n in ("s11", "s12", "s22", "e11", "e12", "e22"):
+        assert rel_l2(a.getDGData(n, S), b.getDGData(n, S)) < tol, n
+
+
 def _module_step_inputs(nx, ny, steps, dt, seed):
     from nextsimdg_amd import inputs
 
