@@ -1367,8 +1367,8 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
     const size_t ne = a.ne;
     const uint32_t off = static_cast<uint32_t>(e) * 8u;
     // The strain of the velocity is computed for every element (the nodes of land elements
-    // exist too) and replaced by the stored one on land: a short branch instead of a long
-    // divergent one around the whole strain computation.
+    // exist too) and replaced by the stored one on land, in a branch taken by whole waves:
+    // no divergent control flow around the strain computation.
     double e11[S], e12[S], e22[S];
     {
         const double* const g1 = phaseLocal(geo);
@@ -1449,10 +1449,20 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
         const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
         inverseMassApply<CG>(g1, ne, offM, t11, t12, t22, e11, e12, e22);
     }
-    if (!a.ocean[e])
-        loadTensorAt<S>(a.eps, ne, off, e11, e12, e22);
+    if (__builtin_amdgcn_ballot_w64(!a.ocean[e]) != 0) { // a wave with land: uniform branch
+        const bool land = !a.ocean[e];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double x = ldAt(a.eps + (0 * S + s) * ne, off), y = ldAt(a.eps + (1 * S + s) * ne, off),
+                         z = ldAt(a.eps + (2 * S + s) * ne, off);
+            e11[s] = land ? x : e11[s];
+            e12[s] = land ? y : e12[s];
+            e22[s] = land ? z : e22[s];
+        }
+    }
     const double* const g2 = phaseLocal(geo);
-    auto gq = [&](int k) { return ldAt(g2 + static_cast<size_t>(k) * ne, off); };
+    const uint32_t offS = after(after(after(off, e11[S - 1]), e12[S - 1]), e22[S - 1]);
+    auto gq = [&](int k) { return ldAt(g2 + static_cast<size_t>(k) * ne, offS); };
     double u11[S], u12[S], u22[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -1469,7 +1479,7 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
             g22 += e22[s] * bd[s];
         }
         const double delta = sqrt(a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12);
-        const double P = ldAt(a.Pg + q * ne, off);
+        const double P = ldAt(a.Pg + q * ne, offS);
         const double c = (P * 0.125) / delta; // P/8/Delta
         const double jw = gq(L::JW + q) * a.invAlpha;
         const double r11 = jw * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
@@ -1508,7 +1518,7 @@ __device__ __forceinline__ void divergenceGeomFrom(const double* geo, size_t ne,
     using L = GeoLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
     const double* const g4 = phaseLocal(geo);
-    const uint32_t off = static_cast<uint32_t>(e) * 8u;
+    const uint32_t off = after(after(after(static_cast<uint32_t>(e) * 8u, s11[S - 1]), s12[S - 1]), s22[S - 1]);
     auto gq = [&](int k) { return ldAt(g4 + static_cast<size_t>(k) * ne, off); };
 #pragma unroll
     for (int j = 0; j < N; ++j)
