@@ -221,6 +221,46 @@ template <int CG> __global__ void k_geometry(MeshView m, int ne, double* geo)
             put(L::mi(s, t), Mi[s][t]);
 }
 
+// Per-element data of the brittle update on general meshes, SoA [k][ne]: the element size
+// h = sqrt(area) (ParametricMesh::area, ParametricMesh.hpp:276-299, of the stored vertices,
+// whose longitudes it does not correct), then iMJwPSI_dam (ParametricMap.cpp:290-296 and
+// 346-351) = M_D^-1 PSI_D diag(w J) at the Gauss points of the stress rule, with the
+// metric (cos-latitude) mass matrix on the sphere.
+template <int CG, int DG> struct BrittleGeo {
+    static constexpr int Q = MomLayout<CG>::Q, H = 0, DAM = 1, K = 1 + DG * Q;
+};
+template <int CG, int DG> __global__ void k_brittle_geometry(MeshView m, int ne, double* out)
+{
+    using L = BrittleGeo<CG, DG>;
+    constexpr int G = MomLayout<CG>::G, Q = L::Q;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    auto put = [&](int k, double v) { out[static_cast<size_t>(k) * ne + e] = v; };
+    {
+        const int nid = (e / m.nx) * (m.nx + 1) + (e % m.nx);
+        const int ids[4] = { nid, nid + 1, nid + m.nx + 1, nid + m.nx + 2 }; // LL, LR, UL, UR
+        auto sq = [&](int i, int j) {
+            const double dx = m.vxy[2 * ids[i]] - m.vxy[2 * ids[j]], dy = m.vxy[2 * ids[i] + 1] - m.vxy[2 * ids[j] + 1];
+            return dx * dx + dy * dy;
+        };
+        const double a = sq(0, 1), b = sq(1, 3), cc = sq(2, 3), d = sq(0, 2), ed = sq(0, 3), f = sq(1, 2);
+        put(L::H, sqrt(0.25 * sqrt(4.0 * ed * f - (b + d - a - cc) * (b + d - a - cc))));
+    }
+    double c[4][2], J[Q], M[DG][DG], Mi[DG][DG];
+    elementCoords(m, e, c);
+    jacobian<G>(c, J);
+    metricMassMatrix<DG>(m, c, M);
+    invert<DG>(M, Mi);
+    for (int d = 0; d < DG; ++d)
+        for (int q = 0; q < Q; ++q) {
+            double a = 0;
+            for (int k = 0; k < DG; ++k)
+                a += Mi[d][k] * Psi<DG, G>::v[k][q] * (W<G>::v[0][q] * J[q]);
+            put(L::DAM + d * Q + q, a);
+        }
+}
+
 // InitializeLumpedCGMassMatrix (ParametricMap.cpp:94-206): node-centric gather of the
 // element integrals of the CG basis (even element rows first, as the stripes add them).
 template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
@@ -1959,6 +1999,12 @@ struct BrittleArgs {
     double dtOverT; // dt / (h_e sqrt(2 (1 + nu) rho_ice)): dt / t_d = dtOverT sqrt(E)
     const double* damageIn;
     double* damage; // SoA [DG][ne]
+    // General meshes: the per-element table of BrittleGeo (null on uniform meshes, whose
+    // cohesion, compr and dtOverT above hold for every element) and the factors that turn
+    // an element's h into them: cohesion = Clab sqrt(0.1 / h), compr = comprStrength
+    // sqrt(0.1 / h), dtOverT = dt / (h tdFac) with tdFac = sqrt(2 (1 + nu) rho_ice).
+    const double* geo;
+    double Clab, comprStrength, tdFac;
 };
 // The BBM arithmetic at one Gauss point (BBMStressUpdateStep.hpp:67-140): y11, y12, y22
 // hold the interpolated stress on entry and the new one on exit, dg the damage. POW5
@@ -2011,17 +2057,26 @@ __device__ __forceinline__ void bbmPoint(const BrittleArgs& b, double hg, double
 // the damage with psi_d w / m_d (exact for the affine elements of a uniform mesh).
 // The new stress goes to acc1..3 and the new damage to accD; with STORE_STRAIN the strain
 // of an ocean element is stored as well.
-template <int CG, int DG, bool STRAIN, bool STORE_STRAIN>
-__device__ __forceinline__ void bbmElement(const ElemArgs& a, const BrittleArgs& b, int e, double* acc1,
+template <int CG, int DG, bool STRAIN, bool STORE_STRAIN, bool UNI>
+__device__ __forceinline__ void bbmElement(const ElemArgs& a, const BrittleArgs& bIn, int e, double* acc1,
     double* acc2, double* acc3, double* accD)
 {
     using L = MomLayout<CG>;
+    using BG = BrittleGeo<CG, DG>;
     constexpr int S = L::S, NG = L::G, Q = L::Q;
     const size_t ne = a.ne;
-    const Ops<true> op { a.ops, a.stride };
+    const Ops<UNI> op { a.ops, a.stride };
+    BrittleArgs b = bIn;
+    if constexpr (!UNI) {
+        // BBMStressUpdateStep.hpp:134, 158-159 with this element's h
+        const double hE = b.geo[BG::H * ne + e], scale = sqrt(0.1 / hE);
+        b.cohesion = b.Clab * scale;
+        b.compr = b.comprStrength * scale;
+        b.dtOverT = b.dt / (hE * b.tdFac);
+    }
     double e11[S], e12[S], e22[S];
     if (STRAIN && a.ocean[e]) {
-        strainLocal<CG, true>(a, e, e11, e12, e22);
+        strainLocal<CG, UNI>(a, e, e11, e12, e22);
         if constexpr (STORE_STRAIN)
             storeTensor<S>(a.epsOut, ne, e, e11, e12, e22);
     } else {
@@ -2070,10 +2125,14 @@ __device__ __forceinline__ void bbmElement(const ElemArgs& a, const BrittleArgs&
             acc3[k] += w * y22;
         }
 #pragma unroll
-        for (int k = 0; k < DG; ++k)
-            accD[k] += (Psi<DG, NG>::v[k][q] * (gauss_weight(NG, q % NG) * gauss_weight(NG, q / NG))
-                           * invMassRef(k))
-                * dg;
+        for (int k = 0; k < DG; ++k) {
+            if constexpr (UNI)
+                accD[k] += (Psi<DG, NG>::v[k][q] * (gauss_weight(NG, q % NG) * gauss_weight(NG, q / NG))
+                               * invMassRef(k))
+                    * dg;
+            else
+                accD[k] += b.geo[static_cast<size_t>(BG::DAM + k * Q + q) * ne + e] * dg;
+        }
     }
 }
 // Factors of the DG basis (dgBasisAt): psi_k(X, Y) = f_FX[k](X) g_GY[k](Y) with f, g in
@@ -2255,7 +2314,7 @@ __device__ __forceinline__ void bbmElementAffine(const ElemArgs& a, const Brittl
     }
 }
 
-template <int CG, int DG, bool STRAIN>
+template <int CG, int DG, bool STRAIN, bool UNI>
 __global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs b)
 {
     constexpr int S = cg2dgstress(CG);
@@ -2263,7 +2322,7 @@ __global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs 
     if (e >= a.ne)
         return;
     double acc1[S], acc2[S], acc3[S], accD[DG];
-    bbmElement<CG, DG, STRAIN, STRAIN>(a, b, e, acc1, acc2, acc3, accD);
+    bbmElement<CG, DG, STRAIN, STRAIN, UNI>(a, b, e, acc1, acc2, acc3, accD);
     storeTensor<S>(a.sig, a.ne, e, acc1, acc2, acc3);
 #pragma unroll
     for (int k = 0; k < DG; ++k)
@@ -2429,7 +2488,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     int precision = NSDG_PREC_FP64;
     size_t ne = 0, nn = 0, nn1 = 0;
     double deltaT = 0;
-    DBuf<double> mops, geo, lumped, lumped1, h, A, ssh, eps, sig;
+    DBuf<double> mops, geo, brittleGeo, lumped, lumped1, h, A, ssh, eps, sig;
     DBuf<double> u, v, u0, v0, uo, vo, ua, va, cgH, cgA, gX, gY, dSx, dSy;
     DBuf<double> tmpDG, tmpDG2, tmpCG, tmpCG2, cg1a, cg1b, cg1c, scratch, dg0a, dg0b, part, res;
     DBuf<uint8_t> dirNode;
@@ -2567,7 +2626,6 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // ice (damage 1), zero averaged velocity, the stress transport on the stress space.
     void initBrittle()
     {
-        require(mesh.uniform, "the brittle rheology needs a uniform (affine) mesh", NSDG_ERR_UNSUPPORTED);
         stressTr = makeTransport(S, mesh, s, prof);
         stressTr->setScheme(2);
         cg2dgS.build(CG, S, mesh, s);
@@ -2590,6 +2648,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         };
         const double a = sq(0, 1), b = sq(1, 3), cc = sq(2, 3), d = sq(0, 2), e = sq(0, 3), f = sq(1, 2);
         hElem = std::sqrt(0.25 * std::sqrt(4.0 * e * f - (b + d - a - cc) * (b + d - a - cc)));
+        if (!mesh.uniform) {
+            // element sizes and damage projections per element
+            brittleGeo.alloc(static_cast<size_t>(BrittleGeo<CG, DG>::K) * ne);
+            k_brittle_geometry<CG, DG><<<blocksFor(ne), kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), brittleGeo.p);
+            checkLaunch("brittle_geometry");
+        }
     }
     void setBrittle(const nsdg_meb_params& m) override
     {
@@ -3082,12 +3146,19 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         b.dtOverT = deltaT / (hElem * std::sqrt(2.0 * (1.0 + meb.nu0) * p.rho_ice));
         b.damageIn = damage.p;
         b.damage = damage.p;
+        b.geo = mesh.uniform ? nullptr : brittleGeo.p;
+        b.Clab = meb.C_lab;
+        b.comprStrength = meb.compr_strength;
+        b.tdFac = std::sqrt(2.0 * (1.0 + meb.nu0) * p.rho_ice);
         return b;
     }
     template <bool STRAIN> void brittleStressAsync()
     {
         Timed t(prof, s, "bbm_element");
-        k_bbm_element<CG, DG, STRAIN><<<blocksFor(ne), kBlock, 0, s>>>(elemArgs(), brittleArgs());
+        if (mesh.uniform)
+            k_bbm_element<CG, DG, STRAIN, true><<<blocksFor(ne), kBlock, 0, s>>>(elemArgs(), brittleArgs());
+        else
+            k_bbm_element<CG, DG, STRAIN, false><<<blocksFor(ne), kBlock, 0, s>>>(elemArgs(), brittleArgs());
         checkLaunch("bbm_element");
     }
     BrittleNode brittleNode() const
@@ -3125,11 +3196,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         swapBuf(v, vAlt);
     }
     // Substeps: one pass each on uniform meshes without periodic segments (the strain of
-    // the last substep's velocity stored before it), else the reference's pass sequence.
+    // the last substep's velocity stored before it), else the reference's pass sequence
+    // (general meshes: per-element operators, sizes and damage projections).
     void brittleSubstepsAsync(int n)
     {
         for (int k = 0; k < n; ++k) {
-            if (fusedSubcycle()) {
+            if (mesh.uniform && !mesh.hasPeriodic) {
                 if (k == n - 1)
                     strainAsync();
                 brittleFusedPass();

@@ -916,9 +916,14 @@ struct MEBParams {
 // side lengths a (lower), b (right), c (top), d (left) and diagonals e, f.
 inline double elementSize(const Mesh& m, size_t eid)
 {
-    double c[4][2]; // LL, LR, UL, UR
-    m.coordinatesOfElement(eid, c);
-    auto sq = [&](int i, int j) { return (c[i][0] - c[j][0]) * (c[i][0] - c[j][0]) + (c[i][1] - c[j][1]) * (c[i][1] - c[j][1]); };
+    // ParametricMesh::area reads the stored vertices as they are, without the longitude
+    // correction of coordinatesOfElement
+    const size_t nid = (eid / m.nx) * (m.nx + 1) + (eid % m.nx);
+    const size_t ids[4] = { nid, nid + 1, nid + m.nx + 1, nid + m.nx + 2 }; // LL, LR, UL, UR
+    auto sq = [&](int i, int j) {
+        const double dx = m.vx[ids[i]] - m.vx[ids[j]], dy = m.vy[ids[i]] - m.vy[ids[j]];
+        return dx * dx + dy * dy;
+    };
     const double a = sq(0, 1), b = sq(1, 3), cc = sq(2, 3), d = sq(0, 2), e = sq(0, 3), f = sq(1, 2);
     return std::sqrt(0.25 * std::sqrt(4.0 * e * f - (b + d - a - cc) * (b + d - a - cc)));
 }

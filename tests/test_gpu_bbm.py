@@ -23,16 +23,27 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def _pair(cg, dg, nx, ny, coords, mask, params=None, meb=None):
+def _pair(cg, dg, nx, ny, coords, mask, params=None, meb=None, spherical=False):
     import nextsimdg_amd as nd
 
     p = dict(DEFAULT_PARAMS, **(params or {}))
     m = dict(MEB_DEFAULTS, **(meb or {}))
     o = OracleDyn(cg, dg, p, bbm=True, meb=m)
-    o.initialise(nx, ny, coords, mask)
+    o.initialise(nx, ny, coords, mask, spherical=spherical)
     g = nd.BBMDynamicsKernel(cg, dg, params=nd.default_params(**p), meb=nd.default_meb_params(**m))
-    g.initialise(coords, False, mask, nx, ny)
+    g.initialise(coords, spherical, mask, nx, ny)
     return o, g
+
+
+def _general_coords(mesh, nx, ny):
+    """A non-affine Cartesian mesh (cells of ~3 km) or a spherical longitude-latitude patch."""
+    from nextsimdg_amd import inputs
+
+    if mesh == "sphere":
+        return inputs.sphere_patch_coords(nx, ny)
+    c = box_coords(nx, ny, 3000.0, 2600.0, skew=0.12)
+    c[:, 0] += 150.0 * np.sin(c[:, 1] / 9000.0)
+    return c
 
 
 def _box_mask(nx, ny):
@@ -110,6 +121,92 @@ def test_bbm_single_passes_match_oracle(cg, dg, turning):
         assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-12, n
     for n in ("uiostress", "viostress"):
         assert rel_l2(g.getDGData(n, dg), o.get_field(n, dg)) < 1e-12, n
+
+
+@pytest.mark.parametrize("mesh", ["general", "sphere"])
+@pytest.mark.parametrize("cg,dg", CASES)
+def test_bbm_single_passes_on_general_meshes(cg, dg, mesh):
+    """The brittle passes with per-element operators, element sizes (h of the stored
+    vertices: radians on the sphere, as the reference computes it) and damage projections."""
+    _say(f"BBM single passes cg={cg} dg={dg} mesh={mesh}")
+    nx, ny = 21, 13
+    o, g = _pair(cg, dg, nx, ny, _general_coords(mesh, nx, ny), _box_mask(nx, ny),
+                 params={"ocean_turning_angle": 25.0}, spherical=mesh == "sphere")
+    S = o.S
+    # stresses near the cohesion C_lab sqrt(0.1 / h) h: about 11.5 kPa for 3 km cells,
+    # 3 MPa for cells of 0.04 rad
+    sscale = 3e6 if mesh == "sphere" else 5e3
+    _set_state(o, g, np.random.default_rng(17 + cg), S, dg, uscale=1e-3, sscale=sscale)
+    o.prepare_iteration()
+    g.prepareIteration()
+    o.begin_iteration(2.0)
+    g.begin_iteration(2.0)
+    o.run_pass("strain")
+    g.projectVelocityToStrain()
+    for n in ("e11", "e12", "e22"):
+        assert rel_l2(g.getDGData(n, S), o.get_field(n, S)) < 1e-12, n
+    d0 = o.get_field("damage", dg).copy()
+    o.run_pass("stress", 2)
+    g.stressUpdateHighOrder(2)
+    d1 = o.get_field("damage", dg)
+    damaged = d1[:, 0] < d0[:, 0] - 1e-9
+    _say(f"  damaged elements: {damaged.sum()} of {o.ne}")
+    assert damaged.any()
+    for n in ("s11", "s12", "s22"):
+        assert rel_l2(g.getDGData(n, S), o.get_field(n, S)) < 1e-12, n
+    assert rel_l2(g.getDGData("damage", dg), d1) < 1e-12
+    o.run_pass("divergence")
+    g.stressDivergence()
+    for n in ("dStressX", "dStressY"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-12, n
+    o.run_pass("momentum")
+    g.updateMomentum()
+    o.run_pass("boundaries")
+    g.applyBoundaries()
+    for n in ("u", "v", "avgU", "avgV"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-12, n
+
+
+@pytest.mark.parametrize("mesh", ["general", "sphere"])
+def test_bbm_model_steps_on_general_meshes(mesh):
+    """Model steps (advection, stress and damage transport, the substeps) under a steady
+    wind: 3 steps of 100 substeps on a non-affine mesh, <= 1e-10 per field.
+
+    On the sphere the reference projects the Gauss-point stress and damage with the
+    cos-latitude mass matrix against Gauss weights without cos(latitude) (iMJwPSI and
+    iMJwPSI_dam, ParametricMap.cpp:333-351), so every BBM substep, which re-projects the
+    whole stress and damage, scales them by about 1/cos(latitude): the brittle state
+    grows geometrically whatever the time step (the mEVP stress update projects only the
+    new viscous-plastic part and stays bounded). The comparison there runs 2 steps of 5
+    substeps, while the state is still finite, at the same bound."""
+    _say(f"BBM steps mesh={mesh}")
+    cg, dg = 2, 6
+    nx, ny = 20, 16
+    steps, nsub = (3, 100) if mesh == "general" else (2, 5)
+    mask = _box_mask(nx, ny)
+    o, g = _pair(cg, dg, nx, ny, _general_coords(mesh, nx, ny), mask, params={"nSteps": nsub},
+                 spherical=mesh == "sphere")
+    rng = np.random.default_rng(4)
+    ne = nx * ny
+    for name, v in (("hice", rng.uniform(0.5, 1.5, ne)), ("cice", rng.uniform(0.85, 1.0, ne)),
+                    ("uwind", rng.normal(8, 2, ne)), ("vwind", rng.normal(-4, 2, ne)),
+                    ("uocean", 0.02 * np.ones(ne)), ("vocean", np.zeros(ne)), ("ssh", np.zeros(ne))):
+        o.set_field(name, v, 1)
+        g.setData(name, v, 1)
+    for k in range(steps):
+        o.update(120.0)
+        g.update(120.0)
+        _say(f"  step {k}: du={rel_l2(g.get_cg('u'), o.get_cg('u')):.2e}")
+    S = o.S
+    got = _fields(g, dg, S, lambda x, n: x.get_cg(n), lambda x, n, c: x.getDGData(n, c))
+    ref = _fields(o, dg, S, lambda x, n: x.get_cg(n), lambda x, n, c: x.get_field(n, c))
+    errs = {n: rel_l2(got[n], ref[n]) for n in got}
+    _say("  " + " ".join(f"{n}={e:.1e}" for n, e in errs.items()))
+    for n in ref:
+        assert np.isfinite(ref[n]).all(), n
+    assert np.abs(got["u"]).max() > 1e-4
+    for n, e in errs.items():
+        assert e < 1e-10, (n, e)
 
 
 @pytest.mark.parametrize("cg,dg", CASES)

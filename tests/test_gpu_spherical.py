@@ -144,7 +144,9 @@ def test_spherical_ten_steps_match_oracle(patch):
         assert e < 1e-10, (n, e)
 
 
-def test_spherical_meshes_run_fp64_mevp_only():
+def test_spherical_meshes_run_fp64_only():
+    """The FP32 variant is built for uniform meshes; both rheologies run on the sphere
+    (BBM parity on spherical patches is in test_gpu_bbm.py)."""
     import nextsimdg_amd as nd
 
     nx, ny = 12, 10
@@ -154,5 +156,6 @@ def test_spherical_meshes_run_fp64_mevp_only():
     with pytest.raises(nd.NsdgError):
         k.set_precision(nd.PREC_FP32)
     b = nd.BBMDynamicsKernel(2, 6)
+    b.initialise(coords, True, mask, nx, ny)
     with pytest.raises(nd.NsdgError):
-        b.initialise(coords, True, mask, nx, ny)
+        b.set_precision(nd.PREC_FP32)

@@ -14,7 +14,11 @@ test, so each part is checked against the mathematics it discretises:
     weak form, -(d_lat phi_j cos(lat), eta) / (phi_j, cos(lat)), is that of
     d_lat(eta cos(lat)) / cos(lat): the gradient plus -tan(lat) eta / R, which this pins;
   * transport: the cos-weighted DG0 mass, computed here by an independent quadrature, is
-    conserved in a general flow, and a constant stays constant under a rigid zonal flow.
+    conserved in a general flow, and a constant stays constant under a rigid zonal flow;
+  * BBM projection: the reference projects Gauss-point values with the cos-latitude mass
+    matrix against weights without cos(latitude) (ParametricMap.cpp:333-351), so a
+    constant damage comes back scaled by about 1/cos(latitude); pinned against that
+    product computed here.
 
 Meshes are built in the rotated (internal) frame: the vertices handed to the kernel are
 those of a longitude-latitude rectangle taken back through the inverse rotation
@@ -257,3 +261,42 @@ def test_spherical_transport_moves_with_the_rotation_rate():
     for _ in range(steps):
         o.advection(dt)
     assert centroid() - c0 == pytest.approx(om * dt * steps, rel=0.02)
+
+
+def test_spherical_bbm_projection_has_no_metric_weight():
+    """One BBM stress pass with no strain and no stress leaves a constant damage d at every
+    Gauss point (no failure: tau + tan(phi) sn = 0), and projects it with iMJwPSI_dam =
+    M_cos^-1 PSI diag(w J): here J = dlon dlat on the longitude-latitude rectangle and
+    M_cos the 3 x 3 Gauss mass matrix with cos(latitude), both computed independently.
+    Cell means come out near d / cos(latitude), not d."""
+    from oracle import MEB_DEFAULTS
+
+    nx, ny, dg, d = 6, 4, 6, 0.5
+    o = OracleDyn(2, dg, dict(DEFAULT_PARAMS), bbm=True, meb=dict(MEB_DEFAULTS))
+    o.initialise(nx, ny, _mesh(nx, ny), np.ones(nx * ny), spherical=True)
+    ne = nx * ny
+    dmg = np.zeros((ne, dg))
+    dmg[:, 0] = d
+    o.set_field("damage", dmg, dg)
+    o.set_field("hice", np.ones(ne), 1)
+    o.set_field("cice", np.ones(ne), 1)
+    o.prepare_iteration()
+    o.begin_iteration(1.2)
+    o.run_pass("stress", 1)
+    got = o.get_field("damage", dg)
+    g, w = np.polynomial.legendre.leggauss(3)
+    t, w = 0.5 * (g + 1), 0.5 * w
+    X, Y = np.meshgrid(t - 0.5, t - 0.5, indexing="ij")  # [x][y]
+    W = np.outer(w, w)
+    basis = np.array([np.ones_like(X), X, Y, X * X - 1 / 12, Y * Y - 1 / 12, X * Y])
+    dlon, dlat = (LON1 - LON0) / nx, (LAT1 - LAT0) / ny
+    J = dlon * dlat
+    for e in range(ne):
+        lat = LAT0 + (e // nx + t[None, :]) * dlat + 0 * X
+        M = np.einsum("iab,jab,ab->ij", basis, basis, W * J * np.cos(lat))
+        rhs = np.einsum("iab,ab->i", basis, W * J * d)
+        want = np.linalg.solve(M, rhs)
+        np.testing.assert_allclose(got[e], want, rtol=1e-12, atol=1e-14)
+        latc = LAT0 + (e // nx + 0.5) * dlat
+        assert abs(got[e, 0] * np.cos(latc) / d - 1.0) < 5e-3
+
