@@ -89,16 +89,16 @@ def measure(nx, ny, steps, warmup, dt=120.0, seed=0, prec=0, kind="plain"):
 def geometry_byte_model(nx, ny, spherical=True):
     """Algorithmic HBM bytes of one one-pass subcycle on a general mesh (CG2): per element
     the stress read and written (3 S coefficients each), the strength at the Q Gauss points
-    and the geometry table (a, b, c, d and J w at each Gauss point, J w sin(lat) / R on the
-    sphere, the upper triangle of the inverse mass matrix); per node u, v read and written
-    and the 7 frozen values read."""
+    and the geometry table (the 6 bilinear map coefficients, on the sphere cos and sin of
+    the latitude at the Q Gauss points, the upper triangle of the inverse mass matrix); per
+    node u, v read and written and the 7 frozen values read."""
     S, Q = 8, 9
     ne, nn = nx * ny, (2 * nx + 1) * (2 * ny + 1)
-    geo = (6 if spherical else 5) * Q + S * (S + 1) // 2
+    geo = 6 + (2 * Q if spherical else 0) + S * (S + 1) // 2
     return 8 * (6 * S + Q + geo) * ne + 88 * nn
 
 
-def spherical_leg(nx, steps, warmup, dt=120.0, seed=0):
+def spherical_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None):
     """The mEVP step on a spherical longitude-latitude patch (per-element geometry with the
     metric terms streamed from HBM): throughput and the one-pass subcycle kernel against
     its byte model."""
@@ -133,7 +133,8 @@ def spherical_leg(nx, steps, warmup, dt=120.0, seed=0):
     avg = tot / c
     out["kernels"]["mevp_fused_geom"] = {"avg_launch_ms": round(avg, 5), "algorithmic_bytes_per_launch": alg,
                                          "achieved": round(alg / (avg * 1e-3) / 1e9, 1), "unit": "GB/s",
-                                         "hbm_frac": round(alg / (avg * 1e-3) / HBM_PEAK, 4)}
+                                         "hbm_frac": round(alg / (avg * 1e-3) / HBM_PEAK, 4),
+                                         "traffic": _traffic(traffic_path, f"mevp_fused_geom_{nx}")}
     out["kernels_ms"] = {n: (round(v[1] / v[0], 5) if v[0] else None) for n, v in stats.items()}
     return out
 
@@ -418,7 +419,7 @@ def main():
         out["bbm"] = brittle_leg(args.bbm_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
     if args.spherical_nx:
         log(f"bench: spherical patch at {args.spherical_nx}^2 ...")
-        out["spherical"] = spherical_leg(args.spherical_nx, max(2, args.steps // 2), 1)
+        out["spherical"] = spherical_leg(args.spherical_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:

@@ -131,13 +131,18 @@ int nsdg_meb_params_default(nsdg_meb_params* p);
  * brittle substeps of dt / nsteps (strain, BBM stress and damage update, divergence,
  * semi-implicit momentum, Dirichlet zero). Extra fields: "damage" (DG, set_field /
  * get_field; starts undamaged, 1), "avgU", "avgV" (get_cg). The ice-ocean stress comes from
- * the averaged velocity. Uniform meshes only (NSDG_ERR_UNSUPPORTED otherwise). */
+ * the averaged velocity. Any mesh: uniform meshes without periodic segments run one
+ * kernel per substep, other meshes the split passes with per-element operators, element
+ * sizes and damage projections (spherical meshes as the reference projects them, with
+ * the cos-latitude mass matrix against weights without cos(latitude)). */
 int nsdg_bbm_create(int cg_degree, int dg_adv, const nsdg_params* params, const nsdg_meb_params* meb, nsdg_dyn** out);
 /* initialise(coords, isSpherical, mask) (DynamicsKernel.hpp:44-75,
  * CGDynamicsKernel.cpp:23-51). Builds the mesh, its Dirichlet lists, the transport and
  * momentum operators on the device. Stress and strain start at zero. With spherical != 0
  * the coordinates are (longitude, latitude) in radians (the module converts degrees,
- * MEVPDynamics.cpp:46-48); such meshes run the per-element operator path (FP64, mEVP). */
+ * MEVPDynamics.cpp:46-48). Non-uniform and spherical meshes (fewer than 2^29 elements)
+ * run the FP64 subcycle from per-element geometry tables; the FP32 variant stays with
+ * uniform meshes. */
 int nsdg_dyn_initialise(nsdg_dyn* d, size_t nx, size_t ny, const double* coords, int spherical,
     const double* mask);
 /* Initialise from an explicitly built mesh (landmask, Dirichlet and periodic lists as set). */

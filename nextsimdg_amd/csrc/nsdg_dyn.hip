@@ -134,20 +134,24 @@ template <int CG> __global__ void k_momentum_ops(MeshView m, int nOps, double* o
     }
 }
 
-// Per-element geometry of the one-pass subcycle on general meshes, SoA [k][ne]. At each
-// Gauss point q of the stress rule: the weighted map gradients that make J grad(phi),
+// Per-element geometry of the one-pass subcycle on general meshes, SoA [k][ne]: the
+// bilinear map x(xi, eta) = x0 + X1 xi + X2 eta + X3 xi eta (and y), so that at any point
+// x_xi = X1 + X3 eta, x_eta = X2 + X3 xi; on the sphere cos and sin of the latitude at the
+// Gauss points of the stress rule; last the upper triangle of the element's inverse
+// (metric) mass matrix. At a Gauss point (gaussGeom) the weighted map gradients that make
+// J grad(phi),
 //   J dphi/dx = phi_xi y_eta - phi_eta y_xi,  J dphi/dy = phi_eta x_xi - phi_xi x_eta,
-// as a = y_eta w, b = y_xi w, c = x_xi w cos(lat), d = x_eta w cos(lat) (all / R on the
-// sphere), then J w and, on the sphere, J w sin(lat) / R; last the upper triangle of the
-// element's inverse (metric) mass matrix. Every operator of InitializeDivSMatrices
-// (ParametricMap.cpp:209-356) factors through these:
+// are a = y_eta w, b = y_xi w, c = x_xi w cos(lat), d = x_eta w cos(lat) (all / R on the
+// sphere), with J w and, on the sphere, J w sin(lat) / R. Every operator of
+// InitializeDivSMatrices (ParametricMap.cpp:209-356) factors through these:
 //   divS1 = sum_q (phi_xi a - phi_eta b) PSI(q),  divS2 = sum_q (phi_eta c - phi_xi d) PSI(q),
-//   divM = sum_q phi JS PSI(q),  iMgradX/Y = M^-1 divS1/2^T,  iMM = M^-1 divM^T,
+//   divM = sum_q phi J w sin(lat) / R PSI(q),  iMgradX/Y = M^-1 divS1/2^T,  iMM = M^-1 divM^T,
 //   iMJwPSI = M^-1 PSI diag(J w),
-// so 6 Q + S (S + 1) / 2 values (90 for CG2) replace the 4 S N + 2 S Q tables (432).
+// so 6 + 2 Q + S (S + 1) / 2 values (60 for CG2 on the sphere, 42 on a plane) replace the
+// 4 S N + 2 S Q tables (432).
 // The geometry pointer as a value the compiler cannot see through: each phase of the
-// element update re-reads the values it needs (L1/L2 hits) instead of keeping every value
-// of the previous phases live in registers.
+// element update re-reads the values it needs instead of keeping every value of the
+// previous phases live in registers.
 __device__ __forceinline__ const double* phaseLocal(const double* p)
 {
     asm volatile("" : "+s"(p));
@@ -155,18 +159,73 @@ __device__ __forceinline__ const double* phaseLocal(const double* p)
 }
 template <int CG> struct GeoLayout {
     static constexpr int S = cg2dgstress(CG), G = gp1d(S), Q = G * G;
-    static constexpr int A = 0, B = Q, C = 2 * Q, D = 3 * Q, JW = 4 * Q, JS = 5 * Q, MI = 6 * Q;
+    static constexpr int X1 = 0, X2 = 1, X3 = 2, Y1 = 3, Y2 = 4, Y3 = 5, CL = 6, SL = 6 + Q, MI = 6 + 2 * Q;
     static constexpr int K = MI + S * (S + 1) / 2;
     __host__ __device__ static constexpr int mi(int s, int t)
     {
         return s <= t ? MI + s * S - s * (s - 1) / 2 + (t - s) : mi(t, s);
     }
 };
+template <int CG> __global__ void k_geometry(MeshView m, int ne, double* geo)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S, G = L::G, Q = L::Q;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    auto put = [&](int k, double v) { geo[static_cast<size_t>(k) * ne + e] = v; };
+    double c[4][2]; // LL, LR, UL, UR
+    elementCoords(m, e, c);
+    put(L::X1, c[1][0] - c[0][0]);
+    put(L::X2, c[2][0] - c[0][0]);
+    put(L::X3, (c[0][0] - c[1][0]) - (c[2][0] - c[3][0]));
+    put(L::Y1, c[1][1] - c[0][1]);
+    put(L::Y2, c[2][1] - c[0][1]);
+    put(L::Y3, (c[0][1] - c[1][1]) - (c[2][1] - c[3][1]));
+    double lat[Q];
+    gaussLatitude<G>(c, lat);
+    for (int q = 0; q < Q; ++q) {
+        put(L::CL + q, m.spherical ? cos(lat[q]) : 1.0);
+        put(L::SL + q, m.spherical ? sin(lat[q]) : 0.0);
+    }
+    double M[S][S], Mi[S][S];
+    metricMassMatrix<S>(m, c, M);
+    invert<S>(M, Mi);
+    for (int s = 0; s < S; ++s)
+        for (int t = s; t < S; ++t)
+            put(L::mi(s, t), Mi[s][t]);
+}
+// The bilinear map coefficients of one element and the geometry they give at a Gauss point.
+struct MapCoef {
+    double x1, x2, x3, y1, y2, y3;
+};
+template <int CG> __device__ __forceinline__ MapCoef loadMap(const double* g, size_t ne, uint32_t off)
+{
+    using L = GeoLayout<CG>;
+    return { ldAt(g + L::X1 * ne, off), ldAt(g + L::X2 * ne, off), ldAt(g + L::X3 * ne, off),
+        ldAt(g + L::Y1 * ne, off), ldAt(g + L::Y2 * ne, off), ldAt(g + L::Y3 * ne, off) };
+}
+struct GaussGeom {
+    double a, b, c, d, jw, js;
+};
+template <int NG, bool SPH>
+__device__ __forceinline__ GaussGeom gaussGeom(const MapCoef& m, int gx, int gy, double cl, double sl)
+{
+    constexpr double iR = SPH ? 1.0 / kEarthRadius : 1.0;
+    const double xi = gauss_point(NG, gx), eta = gauss_point(NG, gy);
+    const double w = gauss_weight(NG, gx) * gauss_weight(NG, gy), wr = w * iR;
+    const double xxi = m.x1 + m.x3 * eta, xeta = m.x2 + m.x3 * xi;
+    const double yxi = m.y1 + m.y3 * eta, yeta = m.y2 + m.y3 * xi;
+    const double J = xxi * yeta - xeta * yxi;
+    const double wc = SPH ? wr * cl : wr;
+    return { yeta * wr, yxi * wr, xxi * wc, xeta * wc, J * w, SPH ? J * w * sl * iR : 0.0 };
+}
 // y_c = M x_c (c = 1, 2, 3) for the element's symmetric inverse mass matrix, stored as its
-// upper triangle: each stored entry is loaded once and applied to both of its positions.
-template <int CG>
-__device__ __forceinline__ void inverseMassApply(const double* g, size_t ne, uint32_t off, const double* x1,
-    const double* x2, const double* x3, double* y1, double* y2, double* y3)
+// upper triangle (entry k = mi(s, t) - MI given by mAt(k)): each stored entry is read once
+// and applied to both of its positions.
+template <int CG, class MA>
+__device__ __forceinline__ void inverseMassApply(const MA& mAt, const double* x1, const double* x2,
+    const double* x3, double* y1, double* y2, double* y3)
 {
     using L = GeoLayout<CG>;
     constexpr int S = L::S;
@@ -177,7 +236,7 @@ __device__ __forceinline__ void inverseMassApply(const double* g, size_t ne, uin
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int t = s; t < S; ++t) {
-            const double m = ldAt(g + static_cast<size_t>(L::mi(s, t)) * ne, off);
+            const double m = mAt(L::mi(s, t) - L::MI);
             y1[s] += m * x1[t];
             y2[s] += m * x2[t];
             y3[s] += m * x3[t];
@@ -188,39 +247,6 @@ __device__ __forceinline__ void inverseMassApply(const double* g, size_t ne, uin
             }
         }
 }
-template <int CG> __global__ void k_geometry(MeshView m, int ne, double* geo)
-{
-    using L = GeoLayout<CG>;
-    constexpr int S = L::S, G = L::G, Q = L::Q;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne)
-        return;
-    auto put = [&](int k, double v) { geo[static_cast<size_t>(k) * ne + e] = v; };
-    const bool sph = m.spherical != 0;
-    const double iR = sph ? 1.0 / kEarthRadius : 1.0;
-    double c[4][2], Fx[2][Q], Fy[2][Q], J[Q], lat[Q];
-    elementCoords(m, e, c);
-    mapGradients<G>(c, Fx, Fy);
-    jacobian<G>(c, J);
-    if (sph)
-        gaussLatitude<G>(c, lat);
-    for (int q = 0; q < Q; ++q) {
-        const double w = W<G>::v[0][q], cl = sph ? cos(lat[q]) : 1.0;
-        put(L::A + q, Fy[1][q] * w * iR);
-        put(L::B + q, Fx[1][q] * w * iR);
-        put(L::C + q, Fx[0][q] * w * cl * iR);
-        put(L::D + q, Fy[0][q] * w * cl * iR);
-        put(L::JW + q, J[q] * w);
-        put(L::JS + q, sph ? J[q] * sin(lat[q]) * w * iR : 0.0);
-    }
-    double M[S][S], Mi[S][S];
-    metricMassMatrix<S>(m, c, M);
-    invert<S>(M, Mi);
-    for (int s = 0; s < S; ++s)
-        for (int t = s; t < S; ++t)
-            put(L::mi(s, t), Mi[s][t]);
-}
-
 // Per-element data of the brittle update on general meshes, SoA [k][ne]: the element size
 // h = sqrt(area) (ParametricMesh::area, ParametricMesh.hpp:276-299, of the stored vertices,
 // whose longitudes it does not correct), then iMJwPSI_dam (ParametricMap.cpp:290-296 and
@@ -1398,9 +1424,11 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
 //    strain evaluated there, then sigma <- (1 - 1/alpha) sigma + M^-1 sum_q PSI(q) J w
 //    sigma_VP(q) / alpha.
 // The same operators as the per-element tables of k_momentum_ops, applied in factored form.
-template <int CG, bool SPH>
+// col(k), k < S (S + 1) / 2: a per-thread scratch column (LDS) that keeps the inverse mass
+// matrix from the strain to the stress projection.
+template <int CG, bool SPH, class COL>
 __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const double* geo, int e, double* s11,
-    double* s12, double* s22)
+    double* s12, double* s22, const COL& col)
 {
     using L = GeoLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
@@ -1412,7 +1440,7 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
     double e11[S], e12[S], e22[S];
     {
         const double* const g1 = phaseLocal(geo);
-        auto gq = [&](int k) { return ldAt(g1 + static_cast<size_t>(k) * ne, off); };
+        const MapCoef map = loadMap<CG>(g1, ne, off);
         double lu[N], lv[N];
         {
             const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
@@ -1464,14 +1492,14 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
                         vq += ph * Iv[jx];
                     }
                 }
-                const double ga = gq(L::A + q), gb = gq(L::B + q), gc = gq(L::C + q), gd = gq(L::D + q);
-                double g11 = ga * uxi - gb * ueta;
-                const double g22 = gc * veta - gd * vxi;
-                double g12 = (ga * vxi - gb * veta) + (gc * ueta - gd * uxi);
+                const GaussGeom gg = gaussGeom<NG, SPH>(map, gx, gy, SPH ? ldAt(g1 + (L::CL + q) * ne, off) : 1.0,
+                    SPH ? ldAt(g1 + (L::SL + q) * ne, off) : 0.0);
+                double g11 = gg.a * uxi - gg.b * ueta;
+                const double g22 = gg.c * veta - gg.d * vxi;
+                double g12 = (gg.a * vxi - gg.b * veta) + (gg.c * ueta - gg.d * uxi);
                 if constexpr (SPH) {
-                    const double js = gq(L::JS + q);
-                    g11 -= js * vq;
-                    g12 += js * uq;
+                    g11 -= gg.js * vq;
+                    g12 += gg.js * uq;
                 }
                 g12 *= 0.5;
                 double bd[8];
@@ -1487,7 +1515,13 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
             }
         }
         const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
-        inverseMassApply<CG>(g1, ne, offM, t11, t12, t22, e11, e12, e22);
+        inverseMassApply<CG>(
+            [&](int k) {
+                const double m = ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM);
+                col(k) = m;
+                return m;
+            },
+            t11, t12, t22, e11, e12, e22);
     }
     if (__builtin_amdgcn_ballot_w64(!a.ocean[e]) != 0) { // a wave with land: uniform branch
         const bool land = !a.ocean[e];
@@ -1502,7 +1536,7 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
     }
     const double* const g2 = phaseLocal(geo);
     const uint32_t offS = after(after(after(off, e11[S - 1]), e12[S - 1]), e22[S - 1]);
-    auto gq = [&](int k) { return ldAt(g2 + static_cast<size_t>(k) * ne, offS); };
+    const MapCoef map2 = loadMap<CG>(g2, ne, offS);
     double u11[S], u12[S], u22[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -1521,7 +1555,7 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
         const double delta = sqrt(a.DM2 + 1.25 * (g11 * g11 + g22 * g22) + 1.50 * g11 * g22 + g12 * g12);
         const double P = ldAt(a.Pg + q * ne, offS);
         const double c = (P * 0.125) / delta; // P/8/Delta
-        const double jw = gq(L::JW + q) * a.invAlpha;
+        const double jw = gaussGeom<NG, false>(map2, q % NG, q / NG, 1.0, 0.0).jw * a.invAlpha;
         const double r11 = jw * (c * (5.0 * g11 + 3.0 * g22) - 0.5 * P);
         const double r12 = jw * ((2.0 * c) * g12);
         const double r22 = jw * (c * (5.0 * g22 + 3.0 * g11) - 0.5 * P);
@@ -1534,10 +1568,10 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
             u22[s] += bd[s] * r22;
         }
     }
-    // the inverse mass and the previous stress are read only now
+    // the inverse mass (from the scratch column) and the previous stress are read only now
     const uint32_t offM = after(after(after(off, u11[S - 1]), u12[S - 1]), u22[S - 1]);
     double x11[S], x12[S], x22[S];
-    inverseMassApply<CG>(phaseLocal(geo), ne, offM, u11, u12, u22, x11, x12, x22);
+    inverseMassApply<CG>([&](int k) { return static_cast<double>(col(k)); }, u11, u12, u22, x11, x12, x22);
     loadTensorAt<S>(a.sig, ne, after(offM, x22[S - 1]), s11, s12, s22);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1559,7 +1593,7 @@ __device__ __forceinline__ void divergenceGeomFrom(const double* geo, size_t ne,
     constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
     const double* const g4 = phaseLocal(geo);
     const uint32_t off = after(after(after(static_cast<uint32_t>(e) * 8u, s11[S - 1]), s12[S - 1]), s22[S - 1]);
-    auto gq = [&](int k) { return ldAt(g4 + static_cast<size_t>(k) * ne, off); };
+    const MapCoef map = loadMap<CG>(g4, ne, off);
 #pragma unroll
     for (int j = 0; j < N; ++j)
         tx[j] = ty[j] = 0.0;
@@ -1581,14 +1615,14 @@ __device__ __forceinline__ void divergenceGeomFrom(const double* geo, size_t ne,
                 v12 += s12[s] * bd[s];
                 v22 += s22[s] * bd[s];
             }
-            const double ga = gq(L::A + q), gb = gq(L::B + q), gc = gq(L::C + q), gd = gq(L::D + q);
-            const double fxXi = ga * v11 - gd * v12, fxEta = gc * v12 - gb * v11;
-            const double fyXi = ga * v12 - gd * v22, fyEta = gc * v22 - gb * v12;
+            const GaussGeom gg = gaussGeom<NG, SPH>(map, gx, gy, SPH ? ldAt(g4 + (L::CL + q) * ne, off) : 1.0,
+                SPH ? ldAt(g4 + (L::SL + q) * ne, off) : 0.0);
+            const double fxXi = gg.a * v11 - gg.d * v12, fxEta = gg.c * v12 - gg.b * v11;
+            const double fyXi = gg.a * v12 - gg.d * v22, fyEta = gg.c * v22 - gg.b * v12;
             double fx0 = 0, fy0 = 0;
             if constexpr (SPH) {
-                const double js = gq(L::JS + q);
-                fx0 = js * v12;
-                fy0 = -js * v11;
+                fx0 = gg.js * v12;
+                fy0 = -gg.js * v11;
             }
 #pragma unroll
             for (int jx = 0; jx < K; ++jx) {
@@ -1620,8 +1654,12 @@ __device__ __forceinline__ void divergenceGeomFrom(const double* geo, size_t ne,
 template <int CG, int DG, bool SPH> __global__ __launch_bounds__(kBlock) void k_mevp_fused_geom(FusedArgs f)
 {
     using T = FusedTile<CG>;
-    constexpr int N = T::N, S = cg2dgstress(CG);
-    __shared__ double contrib[T::HX * T::HY][N][2];
+    constexpr int N = T::N, S = cg2dgstress(CG), NM = S * (S + 1) / 2;
+    // One column per local element: first the inverse mass matrix of its update (rows
+    // < NM), then its divergence contributions (row 2 j + c: node j, component c). Each
+    // thread reads and writes only its own column until the barrier.
+    constexpr int kRows = NM > 2 * N ? NM : 2 * N;
+    __shared__ double stage[kRows][kBlock];
     const NodeArgs& a = f.n;
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
     const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
@@ -1637,20 +1675,20 @@ template <int CG, int DG, bool SPH> __global__ __launch_bounds__(kBlock) void k_
         const bool inside = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
         const int e = min(max(ex, 0), a.nx - 1) + a.nx * min(max(ey, 0), a.ny - 1);
         double s11[S], s12[S], s22[S], tx[N], ty[N];
-        elementGeomUpdate<CG, SPH>(f.e, f.geo, e, s11, s12, s22);
+        elementGeomUpdate<CG, SPH>(f.e, f.geo, e, s11, s12, s22, [&](int k) -> double& { return stage[k][le]; });
         if (inside && lx >= 1 && ly >= 1)
             storeTensorAt<S>(f.sigOut, a.ne, static_cast<uint32_t>(e) * 8u, s11, s12, s22);
         divergenceGeomFrom<CG, SPH>(f.geo, a.ne, e, s11, s12, s22, tx, ty);
         const bool adds = inside && a.ocean[e] != 0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            contrib[le][j][0] = adds ? tx[j] : 0.0;
-            contrib[le][j][1] = adds ? ty[j] : 0.0;
+            stage[2 * j][le] = adds ? tx[j] : 0.0;
+            stage[2 * j + 1][le] = adds ? ty[j] : 0.0;
         }
     }
     __syncthreads();
     nodeTileUpdate<CG, T>(
-        a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
+        a, [&](int le, int j, int c) { return stage[2 * j + c][le]; }, ex0, ey0, f.uOut, f.vOut);
 }
 
 // Split passes (periodic meshes and per-pass testing).
