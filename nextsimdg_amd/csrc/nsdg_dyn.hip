@@ -253,7 +253,13 @@ __device__ __forceinline__ void inverseMassApply(const MA& mAt, const double* x1
 // 346-351) = M_D^-1 PSI_D diag(w J) at the Gauss points of the stress rule, with the
 // metric (cos-latitude) mass matrix on the sphere.
 template <int CG, int DG> struct BrittleGeo {
-    static constexpr int Q = MomLayout<CG>::Q, H = 0, DAM = 1, K = 1 + DG * Q;
+    static constexpr int Q = MomLayout<CG>::Q, H = 0, DAM = 1;
+    // then the upper triangle of the inverse (metric) DG mass matrix, for the one-pass kernel
+    static constexpr int MDI = 1 + DG * Q, K = MDI + DG * (DG + 1) / 2;
+    __host__ __device__ static constexpr int mdi(int s, int t)
+    {
+        return s <= t ? MDI + s * DG - s * (s - 1) / 2 + (t - s) : mdi(t, s);
+    }
 };
 template <int CG, int DG> __global__ void k_brittle_geometry(MeshView m, int ne, double* out)
 {
@@ -285,6 +291,9 @@ template <int CG, int DG> __global__ void k_brittle_geometry(MeshView m, int ne,
                 a += Mi[d][k] * Psi<DG, G>::v[k][q] * (W<G>::v[0][q] * J[q]);
             put(L::DAM + d * Q + q, a);
         }
+    for (int d = 0; d < DG; ++d)
+        for (int k = d; k < DG; ++k)
+            put(L::mdi(d, k), Mi[d][k]);
 }
 
 // InitializeLumpedCGMassMatrix (ParametricMap.cpp:94-206): node-centric gather of the
@@ -2450,6 +2459,7 @@ struct BrittleFusedArgs {
     double* damageOut;
     double* uOut;
     double* vOut;
+    const double* geo; // GeoLayout tables of a general mesh (k_bbm_fused_geom)
 };
 template <int CG, int DG, bool POW5>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_bbm_fused(BrittleFusedArgs f)
@@ -2492,6 +2502,263 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         for (int j = 0; j < N; ++j) {
             stage[2 * j][le] = tx[j];
             stage[2 * j + 1][le] = ty[j];
+        }
+    }
+    __syncthreads();
+    nodeTileUpdate<CG, T, BrittleNode>(
+        a, [&](int le, int j, int c) { return stage[2 * j + c][le]; }, ex0, ey0, f.uOut, f.vOut, f.pol);
+}
+
+// bbmElement for the one-pass kernel on a general mesh, from its geometry (GeoLayout) and
+// brittle table (BrittleGeo): the element's h_e sets cohesion, compressive strength and
+// damage time; the strain is M^-1 sum_q PSI(q) g(q) as in elementGeomUpdate (ocean
+// elements; land elements keep their stored strain) evaluated at the Gauss points; the
+// previous stress and damage are staged at the Gauss points in the thread's column col(r),
+// r = 4 q + {0, 1, 2, 3}, as in bbmElementAffine; the projections are
+// M_S^-1 sum_q PSI(q) J w y(q) and M_D^-1 sum_q PSI_D(q) J w d(q) (iMJwPSI and
+// iMJwPSI_dam, BBMStressUpdateStep.hpp:183-193).
+template <int CG, int DG, bool POW5, bool SPH, class COL>
+__device__ __forceinline__ void bbmElementGeom(const ElemArgs& a, const BrittleArgs& bIn, const double* geo, int e,
+    double* s11, double* s12, double* s22, double* dOut, const COL& col)
+{
+    using L = GeoLayout<CG>;
+    using BG = BrittleGeo<CG, DG>;
+    constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
+    const size_t ne = a.ne;
+    const uint32_t off = static_cast<uint32_t>(e) * 8u;
+    BrittleArgs b = bIn;
+    {
+        // BBMStressUpdateStep.hpp:134, 158-159 with this element's h
+        const double hE = ldAt(b.geo + BG::H * ne, off), scale = sqrt(0.1 / hE);
+        b.cohesion = b.Clab * scale;
+        b.compr = b.comprStrength * scale;
+        b.dtOverT = b.dt / (hE * b.tdFac);
+    }
+    bbmInterpolateToGauss<S, NG>(a.sig + 0 * S * ne, ne, e, 0, col);
+    bbmInterpolateToGauss<S, NG>(a.sig + 1 * S * ne, ne, e, 1, col);
+    bbmInterpolateToGauss<S, NG>(a.sig + 2 * S * ne, ne, e, 2, col);
+    bbmInterpolateToGauss<DG, NG>(b.damageIn, ne, e, 3, col);
+    double e11[S], e12[S], e22[S];
+    {
+        const double* const g1 = phaseLocal(geo);
+        const MapCoef map = loadMap<CG>(g1, ne, off);
+        double lu[N], lv[N];
+        {
+            const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+            const uint32_t noff = static_cast<uint32_t>(CG * row * iy + CG * ix) * 8u;
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    lu[c + K * r] = ldAt(a.u + c + r * row, noff);
+                    lv[c + K * r] = ldAt(a.v + c + r * row, noff);
+                }
+        }
+        double t11[S], t12[S], t22[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            t11[k] = t12[k] = t22[k] = 0.0;
+#pragma unroll
+        for (int gy = 0; gy < NG; ++gy) {
+            double Iu[K], Du[K], Iv[K], Dv[K];
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                double iu = 0, du = 0, iv = 0, dv = 0;
+#pragma unroll
+                for (int jy = 0; jy < K; ++jy) {
+                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+                    iu += ph * lu[jx + K * jy];
+                    du += dp * lu[jx + K * jy];
+                    iv += ph * lv[jx + K * jy];
+                    dv += dp * lv[jx + K * jy];
+                }
+                Iu[jx] = iu;
+                Du[jx] = du;
+                Iv[jx] = iv;
+                Dv[jx] = dv;
+            }
+#pragma unroll
+            for (int gx = 0; gx < NG; ++gx) {
+                const int q = gx + NG * gy;
+                double uxi = 0, ueta = 0, vxi = 0, veta = 0, uq = 0, vq = 0;
+#pragma unroll
+                for (int jx = 0; jx < K; ++jx) {
+                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                    uxi += dp * Iu[jx];
+                    ueta += ph * Du[jx];
+                    vxi += dp * Iv[jx];
+                    veta += ph * Dv[jx];
+                    if constexpr (SPH) {
+                        uq += ph * Iu[jx];
+                        vq += ph * Iv[jx];
+                    }
+                }
+                const GaussGeom gg = gaussGeom<NG, SPH>(map, gx, gy, SPH ? ldAt(g1 + (L::CL + q) * ne, off) : 1.0,
+                    SPH ? ldAt(g1 + (L::SL + q) * ne, off) : 0.0);
+                double g11 = gg.a * uxi - gg.b * ueta;
+                const double g22 = gg.c * veta - gg.d * vxi;
+                double g12 = (gg.a * vxi - gg.b * veta) + (gg.c * ueta - gg.d * uxi);
+                if constexpr (SPH) {
+                    g11 -= gg.js * vq;
+                    g12 += gg.js * uq;
+                }
+                g12 *= 0.5;
+                double bd[8];
+                dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    if (bd[k] == 0.0)
+                        continue;
+                    t11[k] += bd[k] * g11;
+                    t12[k] += bd[k] * g12;
+                    t22[k] += bd[k] * g22;
+                }
+            }
+        }
+        const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
+        inverseMassApply<CG>([&](int k) { return ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM); }, t11, t12,
+            t22, e11, e12, e22);
+    }
+    if (__builtin_amdgcn_ballot_w64(!a.ocean[e]) != 0) { // a wave with land: uniform branch
+        const bool land = !a.ocean[e];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double x = ldAt(a.eps + (0 * S + k) * ne, off), y = ldAt(a.eps + (1 * S + k) * ne, off),
+                         z = ldAt(a.eps + (2 * S + k) * ne, off);
+            e11[k] = land ? x : e11[k];
+            e12[k] = land ? y : e12[k];
+            e22[k] = land ? z : e22[k];
+        }
+    }
+    // the point arithmetic, h and A per Gauss row
+    {
+        const uint32_t offH = after(after(after(off, e11[S - 1]), e12[S - 1]), e22[S - 1]);
+        double hc[DG], ac[DG];
+#pragma unroll
+        for (int i = 0; i < DG; ++i) {
+            hc[i] = ldAt(a.h + i * ne, offH);
+            ac[i] = ldAt(a.A + i * ne, offH);
+        }
+#pragma unroll
+        for (int gy = 0; gy < NG; ++gy) {
+            const double Y = gauss_point(NG, gy) - 0.5;
+            double rh[3] = { 0, 0, 0 }, ra[3] = { 0, 0, 0 };
+#pragma unroll
+            for (int i = 0; i < DG; ++i) {
+                const double g = legendre01(basisGY(i), Y);
+                rh[basisFX(i)] += hc[i] * g;
+                ra[basisFX(i)] += ac[i] * g;
+            }
+#pragma unroll
+            for (int gx = 0; gx < NG; ++gx) {
+                const double X = gauss_point(NG, gx) - 0.5, X2 = X * X - 1.0 / 12.0;
+                double bd[8];
+                dgBasisAt(X, Y, bd);
+                double g11 = 0, g12 = 0, g22 = 0;
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    g11 += e11[k] * bd[k];
+                    g12 += e12[k] * bd[k];
+                    g22 += e22[k] * bd[k];
+                }
+                const int q = gx + NG * gy;
+                const double hg = rh[0] + rh[1] * X + rh[2] * X2, ag = ra[0] + ra[1] * X + ra[2] * X2;
+                double y11 = col(4 * q + 0), y12 = col(4 * q + 1), y22 = col(4 * q + 2), dg = col(4 * q + 3);
+                bbmPoint<POW5>(b, hg, ag, dg, g11, g12, g22, y11, y12, y22);
+                col(4 * q + 0) = y11;
+                col(4 * q + 1) = y12;
+                col(4 * q + 2) = y22;
+                col(4 * q + 3) = dg;
+            }
+        }
+    }
+    // projections
+    const double* const g2 = phaseLocal(geo);
+    const MapCoef map2 = loadMap<CG>(g2, ne, after(off, static_cast<double>(col(4 * NG * NG - 1))));
+    double u11[S], u12[S], u22[S], uD[DG];
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        u11[k] = u12[k] = u22[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < DG; ++k)
+        uD[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NG * NG; ++q) {
+        double bd[8];
+        dgBasisAt(gauss_point(NG, q % NG) - 0.5, gauss_point(NG, q / NG) - 0.5, bd);
+        const double jw = gaussGeom<NG, false>(map2, q % NG, q / NG, 1.0, 0.0).jw;
+        const double y11 = jw * col(4 * q + 0), y12 = jw * col(4 * q + 1), y22 = jw * col(4 * q + 2),
+                     dg = jw * col(4 * q + 3);
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            if (bd[k] == 0.0)
+                continue;
+            u11[k] += bd[k] * y11;
+            u12[k] += bd[k] * y12;
+            u22[k] += bd[k] * y22;
+        }
+#pragma unroll
+        for (int k = 0; k < DG; ++k) {
+            if (bd[k] == 0.0)
+                continue;
+            uD[k] += bd[k] * dg;
+        }
+    }
+    const uint32_t offM = after(after(after(off, u11[S - 1]), u12[S - 1]), u22[S - 1]);
+    inverseMassApply<CG>([&](int k) { return ldAt(g2 + static_cast<size_t>(L::MI + k) * ne, offM); }, u11, u12, u22,
+        s11, s12, s22);
+#pragma unroll
+    for (int k = 0; k < DG; ++k)
+        dOut[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < DG; ++i)
+#pragma unroll
+        for (int k = i; k < DG; ++k) {
+            const double m = ldAt(b.geo + static_cast<size_t>(BG::mdi(i, k)) * ne, offM);
+            dOut[i] += m * uD[k];
+            if (k != i)
+                dOut[k] += m * uD[i];
+        }
+}
+// One whole brittle substep in one pass over a general mesh without periodic segments:
+// k_bbm_fused's tiles, halo, staging and node phase with the element update and the
+// divergence computed from the streamed geometry.
+template <int CG, int DG, bool POW5, bool SPH> __global__ __launch_bounds__(kBlock) void k_bbm_fused_geom(BrittleFusedArgs f)
+{
+    using T = FusedTile<CG>;
+    constexpr int N = T::N, S = cg2dgstress(CG), Q = MomLayout<CG>::Q;
+    // one column per local element: the Gauss-point staging (rows 4 q + c), then the
+    // divergence contributions (row 2 j + c)
+    constexpr int kRows = 4 * Q > 2 * N ? 4 * Q : 2 * N;
+    __shared__ double stage[kRows][kBlock];
+    const NodeArgs& a = f.n;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+    {
+        const int le = threadIdx.x; // one element of tile + halo per thread
+        const int lx = le % T::HX, ly = le / T::HX;
+        const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
+        const bool inside = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
+        const int e = min(max(ex, 0), a.nx - 1) + a.nx * min(max(ey, 0), a.ny - 1);
+        double s11[S], s12[S], s22[S], d[DG], tx[N], ty[N];
+        bbmElementGeom<CG, DG, POW5, SPH>(f.e, f.b, f.geo, e, s11, s12, s22, d,
+            [&](int r) -> double& { return stage[r][le]; });
+        if (inside && lx >= 1 && ly >= 1) {
+            const uint32_t off = static_cast<uint32_t>(e) * 8u;
+            storeTensorAt<S>(f.sigOut, a.ne, off, s11, s12, s22);
+#pragma unroll
+            for (int k = 0; k < DG; ++k)
+                fieldAt(f.damageOut + static_cast<size_t>(k) * a.ne, off) = d[k];
+        }
+        divergenceGeomFrom<CG, SPH>(f.geo, a.ne, e, s11, s12, s22, tx, ty);
+        const bool adds = inside && a.ocean[e] != 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            stage[2 * j][le] = adds ? tx[j] : 0.0;
+            stage[2 * j + 1][le] = adds ? ty[j] : 0.0;
         }
     }
     __syncthreads();
@@ -3221,25 +3488,36 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         f.damageOut = damageAlt.p;
         f.uOut = uAlt.p;
         f.vOut = vAlt.p;
+        f.geo = geo.p;
         using T = FusedTile<CG>;
         const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-        if (meb.exponent_relaxation_sigma == 5.0 && meb.exponent_compression_factor == 1.5)
+        const bool pow5 = meb.exponent_relaxation_sigma == 5.0 && meb.exponent_compression_factor == 1.5;
+        if (!mesh.uniform) {
+            require(geo.p && brittleGeo.p, "general meshes need their geometry tables", NSDG_ERR_STATE);
+            if (mesh.spherical)
+                pow5 ? k_bbm_fused_geom<CG, DG, true, true><<<grid, kBlock, 0, s>>>(f)
+                     : k_bbm_fused_geom<CG, DG, false, true><<<grid, kBlock, 0, s>>>(f);
+            else
+                pow5 ? k_bbm_fused_geom<CG, DG, true, false><<<grid, kBlock, 0, s>>>(f)
+                     : k_bbm_fused_geom<CG, DG, false, false><<<grid, kBlock, 0, s>>>(f);
+        } else if (pow5) {
             k_bbm_fused<CG, DG, true><<<grid, kBlock, 0, s>>>(f);
-        else
+        } else {
             k_bbm_fused<CG, DG, false><<<grid, kBlock, 0, s>>>(f);
+        }
         checkLaunch("bbm_fused");
         swapBuf(sig, sigAlt);
         swapBuf(damage, damageAlt);
         swapBuf(u, uAlt);
         swapBuf(v, vAlt);
     }
-    // Substeps: one pass each on uniform meshes without periodic segments (the strain of
-    // the last substep's velocity stored before it), else the reference's pass sequence
-    // (general meshes: per-element operators, sizes and damage projections).
+    // Substeps: one pass each on meshes without periodic segments (the strain of the last
+    // substep's velocity stored before it), else the reference's pass sequence with the
+    // per-element operators, sizes and damage projections.
     void brittleSubstepsAsync(int n)
     {
         for (int k = 0; k < n; ++k) {
-            if (mesh.uniform && !mesh.hasPeriodic) {
+            if (!mesh.hasPeriodic) {
                 if (k == n - 1)
                     strainAsync();
                 brittleFusedPass();

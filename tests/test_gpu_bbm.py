@@ -209,22 +209,27 @@ def test_bbm_model_steps_on_general_meshes(mesh):
         assert e < 1e-10, (n, e)
 
 
+@pytest.mark.parametrize("mesh,nx,ny", [("box", 37, 29), ("general", 37, 29), ("sphere", 37, 29),
+                                         ("general", 2, 2), ("general", 32, 8), ("sphere", 63, 15)])
 @pytest.mark.parametrize("cg,dg", CASES)
-def test_bbm_fused_substeps_equal_split_passes(cg, dg):
-    """The one-pass substep kernel reproduces the reference's pass sequence."""
-    _say(f"BBM fused vs split cg={cg} dg={dg}")
+def test_bbm_fused_substeps_equal_split_passes(cg, dg, mesh, nx, ny):
+    """The one-pass substep kernels (literal operators on the uniform box, streamed geometry
+    on a non-affine mesh and on a spherical patch; ragged sizes included) reproduce the
+    reference's pass sequence run with the per-element operator tables."""
+    _say(f"BBM fused vs split cg={cg} dg={dg} mesh={mesh} {nx}x{ny}")
     import nextsimdg_amd as nd
 
-    nx, ny = 37, 29  # not a multiple of the tile
-    coords = box_coords(nx, ny, 2000.0)
-    mask = _box_mask(nx, ny)
+    coords = box_coords(nx, ny, 2000.0) if mesh == "box" else _general_coords(mesh, nx, ny)
+    mask = _box_mask(nx, ny) if nx * ny > 4 else np.ones(nx * ny)
+    sph = mesh == "sphere"
     o = OracleDyn(cg, dg, DEFAULT_PARAMS, bbm=True, meb=MEB_DEFAULTS)
-    o.initialise(nx, ny, coords, mask)
+    o.initialise(nx, ny, coords, mask, spherical=sph)
     ks = []
+    sscale = 3e6 if sph else 5e3
     for _ in range(2):
         k = nd.BBMDynamicsKernel(cg, dg, params=nd.default_params(**DEFAULT_PARAMS))
-        k.initialise(coords, False, mask, nx, ny)
-        _set_state(o, k, np.random.default_rng(11), o.S, dg, uscale=1e-3, sscale=5e3)
+        k.initialise(coords, sph, mask, nx, ny)
+        _set_state(o, k, np.random.default_rng(11), o.S, dg, uscale=1e-3, sscale=sscale)
         k.prepareIteration()
         k.begin_iteration(1.2)
         ks.append(k)
@@ -242,9 +247,11 @@ def test_bbm_fused_substeps_equal_split_passes(cg, dg):
     errs.update({n: rel_l2(a.getDGData(n, o.S), b.getDGData(n, o.S)) for n in ("s11", "s12", "s22", "e11", "e12", "e22")})
     errs["damage"] = rel_l2(a.getDGData("damage", dg), b.getDGData("damage", dg))
     _say("  " + " ".join(f"{n}={e:.1e}" for n, e in errs.items()))
-    assert (b.getDGData("damage", dg)[:, 0] < d0[:, 0] - 1e-9).any()  # failure happened
+    if nx * ny >= 64:  # failure happened (on the tiny meshes it may not)
+        assert (b.getDGData("damage", dg)[:, 0] < d0[:, 0] - 1e-9).any()
+    tol = 1e-12 if mesh == "box" else 1e-11
     for n, e in errs.items():
-        assert e < 1e-12, (n, e)
+        assert np.isfinite(e) and e < tol, (n, e)
 
 
 def _inputs(nx, ny, steps, dt, seed):
