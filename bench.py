@@ -41,14 +41,20 @@ def byte_model(nx, ny, cg=2, we=8):
     return elem, node, fused
 
 
-def make_kernel(nx, ny, seed, prec, kind="plain", perturb=0.0, brittle=False):
+def make_kernel(nx, ny, seed, prec, kind="plain", perturb=0.0, brittle=False, warp=0.0):
+    """The benchmark box; warp > 0 shifts the vertices by warp * sin(y / 9 km) in x, which
+    makes every element a non-affine quadrilateral (the general-mesh kernels)."""
     import numpy as np
     import nextsimdg_amd as nd
     from nextsimdg_amd import inputs
 
     s = inputs.benchmark_setup(nx, ny, seed, kind=kind)
     k = nd.BBMDynamicsKernel(2, 6) if brittle else nd.MEVPDynamicsKernel(2, 6)
-    k.initialise(s["coords"], False, s["mask"], nx, ny)
+    coords = s["coords"]
+    if warp:
+        coords = coords.copy()
+        coords[:, 0] += warp * np.sin(coords[:, 1] / 9000.0)
+    k.initialise(coords, False, s["mask"], nx, ny)
     h = s["h"] * (1.0 + perturb * np.random.default_rng(seed + 777).standard_normal(s["h"].size))
     k.setData("hice", h, 1)
     k.setData("cice", s["A"], 1)
@@ -149,11 +155,20 @@ def brittle_byte_model(nx, ny, cg=2, dg=6):
     return 8 * (6 * S + 4 * dg) * ne + 8 * 17 * nn
 
 
-def brittle_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None):
+def brittle_general_byte_model(nx, ny, cg=2, dg=6):
+    """brittle_byte_model on a general (planar) mesh: plus the geometry table (6 bilinear map
+    coefficients and the upper triangle of the inverse stress mass matrix) and the brittle
+    table (h_e and the upper triangle of the inverse DG mass matrix) per element."""
+    S = 8 if cg == 2 else 3
+    return brittle_byte_model(nx, ny, cg, dg) + 8 * (6 + S * (S + 1) // 2 + 1 + dg * (dg + 1) // 2) * nx * ny
+
+
+def brittle_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None, warp=0.0):
     """The BBM (brittle) dynamics step on the same box and forcing: advection, stress and
     damage transport, 100 substeps of dt / 100. Mcell-BBM-substeps/s and the one-pass
-    substep kernel against its byte model."""
-    k, off, L = make_kernel(nx, nx, seed, 0, brittle=True)
+    substep kernel against its byte model. warp > 0: the same box with non-affine elements
+    (the one-pass kernel from per-element geometry)."""
+    k, off, L = make_kernel(nx, nx, seed, 0, brittle=True, warp=warp)
     run_steps(k, off, L, 0, warmup, dt)
     k.synchronize()
     t = time.perf_counter()
@@ -168,21 +183,23 @@ def brittle_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None):
     k.set_profiling(False)
     c, tot = stats["bbm_fused"]
     avg_ms = tot / c
-    alg = brittle_byte_model(nx, nx)
+    alg = brittle_general_byte_model(nx, nx) if warp else brittle_byte_model(nx, nx)
+    mesh = f"benchmark box {nx}x{nx} with non-affine elements (x += {warp:g} m sin(y / 9 km))" if warp else \
+        f"benchmark box {nx}x{nx}"
     return {
-        "workload": f"BBM brittle dynamics step, benchmark box {nx}x{nx}, CG2 velocity, DG2 advection, "
+        "workload": f"BBM brittle dynamics step, {mesh}, CG2 velocity, DG2 advection, "
                     f"8-dof DG stress, FP64, dt=120 s, 100 substeps of 1.2 s, moving cyclone",
         "value": round(nx * nx * 100 * steps / el / 1e6, 3),
         "unit": "Mcell-BBM-substeps/s",
         "steps": steps,
         "ms_per_step": round(el / steps * 1e3, 4),
-        "kernel": "bbm_fused",
+        "kernel": "bbm_fused_geom" if warp else "bbm_fused",
         "avg_launch_ms": round(avg_ms, 5),
         "algorithmic_bytes_per_launch": alg,
         "achieved": round(alg / (avg_ms * 1e-3) / 1e9, 1),
         "unit_achieved": "GB/s",
         "hbm_frac": round(alg / (avg_ms * 1e-3) / HBM_PEAK, 4),
-        "traffic": _traffic(traffic_path, f"bbm_fused_{nx}"),
+        "traffic": _traffic(traffic_path, f"bbm_fused_geom_{nx}" if warp else f"bbm_fused_{nx}"),
         "kernels_ms": {n: (round(v[1] / v[0], 5) if v[0] else None) for n, v in stats.items()},
     }
 
@@ -343,6 +360,8 @@ def main():
     ap.add_argument("--fp32-error-steps", type=int, default=100)
     ap.add_argument("--learned-nx", type=int, default=1024, help="learned vs plain size (0 skips it)")
     ap.add_argument("--bbm-nx", type=int, default=512, help="brittle (BBM) step size (0 skips it)")
+    ap.add_argument("--bbm-general-nx", type=int, default=512,
+                    help="brittle step on a non-affine mesh of this size (0 skips it)")
     ap.add_argument("--spherical-nx", type=int, default=512, help="spherical patch size (0 skips it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
@@ -417,6 +436,10 @@ def main():
     if args.bbm_nx:
         log(f"bench: BBM brittle step at {args.bbm_nx}^2 ...")
         out["bbm"] = brittle_leg(args.bbm_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
+    if args.bbm_general_nx:
+        log(f"bench: BBM brittle step on a non-affine {args.bbm_general_nx}^2 mesh ...")
+        out["bbm_general_mesh"] = brittle_leg(args.bbm_general_nx, max(2, args.steps // 2), 1,
+                                              traffic_path=args.traffic, warp=50.0)
     if args.spherical_nx:
         log(f"bench: spherical patch at {args.spherical_nx}^2 ...")
         out["spherical"] = spherical_leg(args.spherical_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
