@@ -131,10 +131,11 @@ int nsdg_meb_params_default(nsdg_meb_params* p);
  * brittle substeps of dt / nsteps (strain, BBM stress and damage update, divergence,
  * semi-implicit momentum, Dirichlet zero). Extra fields: "damage" (DG, set_field /
  * get_field; starts undamaged, 1), "avgU", "avgV" (get_cg). The ice-ocean stress comes from
- * the averaged velocity. Any mesh: uniform meshes without periodic segments run one
- * kernel per substep, other meshes the split passes with per-element operators, element
- * sizes and damage projections (spherical meshes as the reference projects them, with
- * the cos-latitude mass matrix against weights without cos(latitude)). */
+ * the averaged velocity. Any mesh: meshes without periodic segments run one kernel per
+ * substep (non-uniform ones from per-element geometry, element sizes and damage mass
+ * matrices), periodic meshes the split passes; spherical meshes are projected as the
+ * reference projects them, with the cos-latitude mass matrix against weights without
+ * cos(latitude). */
 int nsdg_bbm_create(int cg_degree, int dg_adv, const nsdg_params* params, const nsdg_meb_params* meb, nsdg_dyn** out);
 /* initialise(coords, isSpherical, mask) (DynamicsKernel.hpp:44-75,
  * CGDynamicsKernel.cpp:23-51). Builds the mesh, its Dirichlet lists, the transport and
