@@ -397,7 +397,7 @@ def main():
     traffic = _traffic(args.traffic, f"{dom}_{rnx}")
     # % of HBM peak on the two-pass byte model of the reference's formulation (element pass:
     # u, v, h, A, sigma r+w; node pass: sigma and 13 node fields r, u, v w), 8 (9 S + 2 D) per
-    # element plus 8 * 17 per node, FP64: the figure whole-step throughput is judged by.
+    # element plus 8 * 17 per node, FP64: whole-step throughput measured against that model.
     ne, nn = nx * ny, (2 * nx + 1) * (2 * ny + 1)
     ref_bytes = 8 * (9 * 8 + 2 * 6) * ne + 8 * 17 * nn
     out = {
