@@ -29,15 +29,15 @@ def byte_model(nx, ny, cg=2, we=8):
     """Algorithmic HBM bytes of one subcycle: every value a pass needs, moved once.
     Two-pass form: element pass = stress read + written (3 S coefficients each), ice
     strength at the Q Gauss points, u and v at the nodes; node pass = stress read, per node
-    u, v and the 7 frozen momentum coefficients read, u, v written. One-pass (fused) form:
+    u, v and the 6 frozen momentum coefficients read, u, v written. One-pass (fused) form:
     stress read + written, strength, per node u, v and the frozen coefficients read, u, v
     written. Returns (element, node, fused)."""
     S, Q = (8, 9) if cg == 2 else (3, 4)
     ne = nx * ny
     nn = (cg * nx + 1) * (cg * ny + 1)
     elem = we * (6 * S + Q) * ne + 8 * 2 * nn
-    node = we * 3 * S * ne + 8 * 11 * nn
-    fused = we * (6 * S + Q) * ne + 8 * 11 * nn
+    node = we * 3 * S * ne + 8 * 10 * nn
+    fused = we * (6 * S + Q) * ne + 8 * 10 * nn
     return elem, node, fused
 
 
@@ -97,11 +97,11 @@ def geometry_byte_model(nx, ny, spherical=True):
     the stress read and written (3 S coefficients each), the strength at the Q Gauss points
     and the geometry table (the 6 bilinear map coefficients, on the sphere cos and sin of
     the latitude at the Q Gauss points, the upper triangle of the inverse mass matrix); per
-    node u, v read and written and the 7 frozen values read."""
+    node u, v read and written and the 6 frozen values read."""
     S, Q = 8, 9
     ne, nn = nx * ny, (2 * nx + 1) * (2 * ny + 1)
     geo = 6 + (2 * Q if spherical else 0) + S * (S + 1) // 2
-    return 8 * (6 * S + Q + geo) * ne + 88 * nn
+    return 8 * (6 * S + Q + geo) * ne + 80 * nn
 
 
 def spherical_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None):

@@ -1051,7 +1051,6 @@ struct NodeArgs {
     const double2* ocn; // frozen per-node coefficients, see k_node_coeffs
     const double2* cf;
     const double2* qf;
-    const double* iML;
     double* dSx;
     double* dSy;
     double rho_ice, F_atm, F_ocean, fc, gravity, beta, dt;
@@ -1228,11 +1227,13 @@ __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R*
 
 // The momentum update with the terms that stay fixed over the subcycles of one timestep
 // (u0, v0, wind, H, A, SSH gradient, lumped mass) folded once per timestep into
-//   ocn = (uo, vo), cf = (c1, A F_ocean) with c1 = rho_ice H / dt, iML = 1 / ML,
-//   qf = (c1 u0 + A F_atm |ua| ua - rho_ice H g gx, c1 v0 + A F_atm |ua| va - rho_ice H g gy),
-// so that a subcycle reads 9 values per node instead of 13; the update itself is the one
-// of VPCGDynamicsKernel.hpp:132-172, same-component Coriolis term included.
-__global__ __launch_bounds__(kBlock) void k_node_coeffs(NodeArgs a, double2* ocn, double2* cf, double2* qf, double* iML)
+//   ocn = (uo, vo), cf = ML (c1, A F_ocean) with c1 = rho_ice H / dt,
+//   qf = ML (c1 u0 + A F_atm |ua| ua - rho_ice H g gx, c1 v0 + A F_atm |ua| va - rho_ice H g gy),
+// so that a subcycle reads 8 values per node instead of 13. The update is the one of
+// VPCGDynamicsKernel.hpp:132-172 (same-component Coriolis term included) with numerator
+// and denominator multiplied by the lumped mass ML, which then divides nothing: the weak
+// divergence enters as it is.
+__global__ __launch_bounds__(kBlock) void k_node_coeffs(NodeArgs a, double2* ocn, double2* cf, double2* qf)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.nn)
@@ -1240,14 +1241,14 @@ __global__ __launch_bounds__(kBlock) void k_node_coeffs(NodeArgs a, double2* ocn
     const double H = a.H[n], A = a.A[n], ua = a.ua[n], va = a.va[n];
     const double absatm = sqrt(ua * ua + va * va);
     const double c1 = a.rho_ice * H / a.dt, rhoH = a.rho_ice * H;
+    const double ML = a.ML[n];
     ocn[n] = make_double2(a.uo[n], a.vo[n]);
-    cf[n] = make_double2(c1, A * a.F_ocean);
-    qf[n] = make_double2(c1 * a.u0[n] + A * (a.F_atm * absatm * ua) - rhoH * a.gravity * a.gx[n],
-        c1 * a.v0[n] + A * (a.F_atm * absatm * va) - rhoH * a.gravity * a.gy[n]);
-    iML[n] = 1.0 / a.ML[n];
+    cf[n] = make_double2(ML * c1, ML * (A * a.F_ocean));
+    qf[n] = make_double2(ML * (c1 * a.u0[n] + A * (a.F_atm * absatm * ua) - rhoH * a.gravity * a.gx[n]),
+        ML * (c1 * a.v0[n] + A * (a.F_atm * absatm * va) - rhoH * a.gravity * a.gy[n]));
 }
 struct NodeFz {
-    double u, v, iML;
+    double u, v;
     double2 ocn, cf, qf;
 };
 __device__ __forceinline__ void loadFrozen(const NodeArgs& a, int n, NodeFz& in)
@@ -1257,18 +1258,17 @@ __device__ __forceinline__ void loadFrozen(const NodeArgs& a, int n, NodeFz& in)
     in.ocn = a.ocn[n];
     in.cf = a.cf[n];
     in.qf = a.qf[n];
-    in.iML = a.iML[n];
 }
 __device__ __forceinline__ void momentumFrozen(const NodeArgs& a, const NodeFz& in, double dSx, double dSy, double& un,
     double& vn)
 {
     const double uOcnRel = in.ocn.x - in.u, vOcnRel = in.v - in.ocn.y;
     const double absocn = sqrt(uOcnRel * uOcnRel + vOcnRel * vOcnRel);
-    const double c1 = in.cf.x, drag = in.cf.y * absocn;
-    const double cor = c1 * a.dt * a.fc; // rho_ice H fc
+    const double c1 = in.cf.x, drag = in.cf.y * absocn; // both times ML
+    const double cor = c1 * a.dt * a.fc;                 // ML rho_ice H fc
     const double inv = 1.0 / (c1 * (1.0 + a.beta) + drag);
-    un = inv * (c1 * a.beta * in.u + in.qf.x + drag * in.ocn.x - cor * in.u + dSx * in.iML);
-    vn = inv * (c1 * a.beta * in.v + in.qf.y + drag * in.ocn.y + cor * in.v + dSy * in.iML);
+    un = inv * (c1 * a.beta * in.u + in.qf.x + drag * in.ocn.x - cor * in.u + dSx);
+    vn = inv * (c1 * a.beta * in.v + in.qf.y + drag * in.ocn.y + cor * in.v + dSy);
 }
 
 // Node tiles: a block owns the CG nodes of the lower-left 2x2 (CG2) node blocks of TX x TY
@@ -2798,7 +2798,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<double> tmpDG, tmpDG2, tmpCG, tmpCG2, cg1a, cg1b, cg1c, scratch, dg0a, dg0b, part, res;
     DBuf<uint8_t> dirNode;
     DBuf<double2> fzOcn, fzCf, fzQ; // frozen momentum coefficients (k_node_coeffs)
-    DBuf<double> fzIML, strength;   // 1 / ML; ice strength at the Gauss points
+    DBuf<double> strength;          // ice strength at the Gauss points
     DBuf<float> sigF, strengthF;    // FP32 variant: stress and strength
     DBuf<float> sigFAlt;            // ping-pong partner of sigF (fused subcycle)
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
@@ -2873,7 +2873,6 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             b->alloc(nn1);
         for (auto* b : { &fzOcn, &fzCf, &fzQ })
             b->alloc(nn);
-        fzIML.alloc(nn);
         strength.alloc(static_cast<size_t>(L::Q) * ne);
         part.alloc(4 * blocksFor(ne));
         res.alloc(2);
@@ -3225,7 +3224,6 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         a.ocn = fzOcn.p;
         a.cf = fzCf.p;
         a.qf = fzQ.p;
-        a.iML = fzIML.p;
         a.dSx = dSx.p;
         a.dSy = dSy.p;
         a.rho_ice = p.rho_ice;
@@ -3306,7 +3304,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         else
             k_strength<CG, DG, double><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar, strength.p);
         checkLaunch("strength");
-        k_node_coeffs<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), fzOcn.p, fzCf.p, fzQ.p, fzIML.p);
+        k_node_coeffs<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), fzOcn.p, fzCf.p, fzQ.p);
         checkLaunch("node_coeffs");
     }
     // A run of subcycles: the frozen terms once, the FP32 copy of the stress on entry and
