@@ -1424,6 +1424,112 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
         a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
 }
 
+// The strain coefficients of one element's velocity on a general mesh (CGDynamicsKernel.cpp:
+// 300-337, with the spherical metric terms) from its geometry: e = M^-1 sum_q PSI(q) g(q)
+// with g the J-weighted velocity gradients at the Gauss points, sum-factorised from the
+// 1-D CG basis. mAt(g, k, off) reads entry k of the packed inverse mass (and may keep it).
+template <int CG, bool SPH, class MAT>
+__device__ __forceinline__ void strainGeom(const ElemArgs& a, const double* geo, int e, double* e11, double* e12,
+    double* e22, const MAT& mAt)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
+    const size_t ne = a.ne;
+    const uint32_t off = static_cast<uint32_t>(e) * 8u;
+    const double* const g1 = phaseLocal(geo);
+    const MapCoef map = loadMap<CG>(g1, ne, off);
+    double lu[N], lv[N];
+    {
+        const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+        const uint32_t noff = static_cast<uint32_t>(CG * row * iy + CG * ix) * 8u;
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                lu[c + K * r] = ldAt(a.u + c + r * row, noff);
+                lv[c + K * r] = ldAt(a.v + c + r * row, noff);
+            }
+    }
+    double t11[S], t12[S], t22[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        t11[s] = t12[s] = t22[s] = 0.0;
+#pragma unroll
+    for (int gy = 0; gy < NG; ++gy) {
+        double Iu[K], Du[K], Iv[K], Dv[K];
+#pragma unroll
+        for (int jx = 0; jx < K; ++jx) {
+            double iu = 0, du = 0, iv = 0, dv = 0;
+#pragma unroll
+            for (int jy = 0; jy < K; ++jy) {
+                const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
+                iu += ph * lu[jx + K * jy];
+                du += dp * lu[jx + K * jy];
+                iv += ph * lv[jx + K * jy];
+                dv += dp * lv[jx + K * jy];
+            }
+            Iu[jx] = iu;
+            Du[jx] = du;
+            Iv[jx] = iv;
+            Dv[jx] = dv;
+        }
+#pragma unroll
+        for (int gx = 0; gx < NG; ++gx) {
+            const int q = gx + NG * gy;
+            double uxi = 0, ueta = 0, vxi = 0, veta = 0, uq = 0, vq = 0;
+#pragma unroll
+            for (int jx = 0; jx < K; ++jx) {
+                const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
+                uxi += dp * Iu[jx];
+                ueta += ph * Du[jx];
+                vxi += dp * Iv[jx];
+                veta += ph * Dv[jx];
+                if constexpr (SPH) {
+                    uq += ph * Iu[jx];
+                    vq += ph * Iv[jx];
+                }
+            }
+            const GaussGeom gg = gaussGeom<NG, SPH>(map, gx, gy, SPH ? ldAt(g1 + (L::CL + q) * ne, off) : 1.0,
+                SPH ? ldAt(g1 + (L::SL + q) * ne, off) : 0.0);
+            double g11 = gg.a * uxi - gg.b * ueta;
+            const double g22 = gg.c * veta - gg.d * vxi;
+            double g12 = (gg.a * vxi - gg.b * veta) + (gg.c * ueta - gg.d * uxi);
+            if constexpr (SPH) {
+                g11 -= gg.js * vq;
+                g12 += gg.js * uq;
+            }
+            g12 *= 0.5;
+            double bd[8];
+            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (bd[s] == 0.0) // basis functions vanishing at this Gauss point (a literal test)
+                    continue;
+                t11[s] += bd[s] * g11;
+                t12[s] += bd[s] * g12;
+                t22[s] += bd[s] * g22;
+            }
+        }
+    }
+    const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
+    inverseMassApply<CG>([&](int k) { return mAt(g1, k, offM); }, t11, t12, t22, e11, e12, e22);
+}
+// The stored strain of the ocean elements from the geometry (the strain output before the
+// last subcycle of a run on a general mesh); land elements keep theirs.
+template <int CG, bool SPH> __global__ __launch_bounds__(kBlock) void k_strain_geom(ElemArgs a, const double* geo)
+{
+    using L = GeoLayout<CG>;
+    constexpr int S = L::S;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.ne || !a.ocean[e])
+        return;
+    const size_t ne = a.ne;
+    double e11[S], e12[S], e22[S];
+    strainGeom<CG, SPH>(a, geo, e, e11, e12, e22, [&](const double* g1, int k, uint32_t offM) {
+        return ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM);
+    });
+    storeTensorAt<S>(a.epsOut, ne, static_cast<uint32_t>(e) * 8u, e11, e12, e22);
+}
 // The element update of one mEVP subcycle on a general mesh from its geometry (GeoLayout):
 //  * strain (ocean elements): e = M^-1 sum_q PSI(q) g(q), with g the J-weighted velocity
 //    gradients at q, sum-factorised from the 1-D CG basis, and on the sphere the metric
@@ -1447,91 +1553,11 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
     // exist too) and replaced by the stored one on land, in a branch taken by whole waves:
     // no divergent control flow around the strain computation.
     double e11[S], e12[S], e22[S];
-    {
-        const double* const g1 = phaseLocal(geo);
-        const MapCoef map = loadMap<CG>(g1, ne, off);
-        double lu[N], lv[N];
-        {
-            const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
-            const uint32_t noff = static_cast<uint32_t>(CG * row * iy + CG * ix) * 8u;
-#pragma unroll
-            for (int r = 0; r < K; ++r)
-#pragma unroll
-                for (int c = 0; c < K; ++c) {
-                    lu[c + K * r] = ldAt(a.u + c + r * row, noff);
-                    lv[c + K * r] = ldAt(a.v + c + r * row, noff);
-                }
-        }
-        double t11[S], t12[S], t22[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-            t11[s] = t12[s] = t22[s] = 0.0;
-#pragma unroll
-        for (int gy = 0; gy < NG; ++gy) {
-            double Iu[K], Du[K], Iv[K], Dv[K];
-#pragma unroll
-            for (int jx = 0; jx < K; ++jx) {
-                double iu = 0, du = 0, iv = 0, dv = 0;
-#pragma unroll
-                for (int jy = 0; jy < K; ++jy) {
-                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
-                    iu += ph * lu[jx + K * jy];
-                    du += dp * lu[jx + K * jy];
-                    iv += ph * lv[jx + K * jy];
-                    dv += dp * lv[jx + K * jy];
-                }
-                Iu[jx] = iu;
-                Du[jx] = du;
-                Iv[jx] = iv;
-                Dv[jx] = dv;
-            }
-#pragma unroll
-            for (int gx = 0; gx < NG; ++gx) {
-                const int q = gx + NG * gy;
-                double uxi = 0, ueta = 0, vxi = 0, veta = 0, uq = 0, vq = 0;
-#pragma unroll
-                for (int jx = 0; jx < K; ++jx) {
-                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
-                    uxi += dp * Iu[jx];
-                    ueta += ph * Du[jx];
-                    vxi += dp * Iv[jx];
-                    veta += ph * Dv[jx];
-                    if constexpr (SPH) {
-                        uq += ph * Iu[jx];
-                        vq += ph * Iv[jx];
-                    }
-                }
-                const GaussGeom gg = gaussGeom<NG, SPH>(map, gx, gy, SPH ? ldAt(g1 + (L::CL + q) * ne, off) : 1.0,
-                    SPH ? ldAt(g1 + (L::SL + q) * ne, off) : 0.0);
-                double g11 = gg.a * uxi - gg.b * ueta;
-                const double g22 = gg.c * veta - gg.d * vxi;
-                double g12 = (gg.a * vxi - gg.b * veta) + (gg.c * ueta - gg.d * uxi);
-                if constexpr (SPH) {
-                    g11 -= gg.js * vq;
-                    g12 += gg.js * uq;
-                }
-                g12 *= 0.5;
-                double bd[8];
-                dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    if (bd[s] == 0.0) // basis functions vanishing at this Gauss point (a literal test)
-                        continue;
-                    t11[s] += bd[s] * g11;
-                    t12[s] += bd[s] * g12;
-                    t22[s] += bd[s] * g22;
-                }
-            }
-        }
-        const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
-        inverseMassApply<CG>(
-            [&](int k) {
-                const double m = ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM);
-                col(k) = m;
-                return m;
-            },
-            t11, t12, t22, e11, e12, e22);
-    }
+    strainGeom<CG, SPH>(a, geo, e, e11, e12, e22, [&](const double* g1, int k, uint32_t offM) {
+        const double m = ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM);
+        col(k) = m;
+        return m;
+    });
     if (__builtin_amdgcn_ballot_w64(!a.ocean[e]) != 0) { // a wave with land: uniform branch
         const bool land = !a.ocean[e];
 #pragma unroll
@@ -2539,86 +2565,9 @@ __device__ __forceinline__ void bbmElementGeom(const ElemArgs& a, const BrittleA
     bbmInterpolateToGauss<S, NG>(a.sig + 2 * S * ne, ne, e, 2, col);
     bbmInterpolateToGauss<DG, NG>(b.damageIn, ne, e, 3, col);
     double e11[S], e12[S], e22[S];
-    {
-        const double* const g1 = phaseLocal(geo);
-        const MapCoef map = loadMap<CG>(g1, ne, off);
-        double lu[N], lv[N];
-        {
-            const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
-            const uint32_t noff = static_cast<uint32_t>(CG * row * iy + CG * ix) * 8u;
-#pragma unroll
-            for (int r = 0; r < K; ++r)
-#pragma unroll
-                for (int c = 0; c < K; ++c) {
-                    lu[c + K * r] = ldAt(a.u + c + r * row, noff);
-                    lv[c + K * r] = ldAt(a.v + c + r * row, noff);
-                }
-        }
-        double t11[S], t12[S], t22[S];
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-            t11[k] = t12[k] = t22[k] = 0.0;
-#pragma unroll
-        for (int gy = 0; gy < NG; ++gy) {
-            double Iu[K], Du[K], Iv[K], Dv[K];
-#pragma unroll
-            for (int jx = 0; jx < K; ++jx) {
-                double iu = 0, du = 0, iv = 0, dv = 0;
-#pragma unroll
-                for (int jy = 0; jy < K; ++jy) {
-                    const double ph = cg_1d(CG, jy, gauss_point(NG, gy)), dp = cg_1d_dx(CG, jy, gauss_point(NG, gy));
-                    iu += ph * lu[jx + K * jy];
-                    du += dp * lu[jx + K * jy];
-                    iv += ph * lv[jx + K * jy];
-                    dv += dp * lv[jx + K * jy];
-                }
-                Iu[jx] = iu;
-                Du[jx] = du;
-                Iv[jx] = iv;
-                Dv[jx] = dv;
-            }
-#pragma unroll
-            for (int gx = 0; gx < NG; ++gx) {
-                const int q = gx + NG * gy;
-                double uxi = 0, ueta = 0, vxi = 0, veta = 0, uq = 0, vq = 0;
-#pragma unroll
-                for (int jx = 0; jx < K; ++jx) {
-                    const double ph = cg_1d(CG, jx, gauss_point(NG, gx)), dp = cg_1d_dx(CG, jx, gauss_point(NG, gx));
-                    uxi += dp * Iu[jx];
-                    ueta += ph * Du[jx];
-                    vxi += dp * Iv[jx];
-                    veta += ph * Dv[jx];
-                    if constexpr (SPH) {
-                        uq += ph * Iu[jx];
-                        vq += ph * Iv[jx];
-                    }
-                }
-                const GaussGeom gg = gaussGeom<NG, SPH>(map, gx, gy, SPH ? ldAt(g1 + (L::CL + q) * ne, off) : 1.0,
-                    SPH ? ldAt(g1 + (L::SL + q) * ne, off) : 0.0);
-                double g11 = gg.a * uxi - gg.b * ueta;
-                const double g22 = gg.c * veta - gg.d * vxi;
-                double g12 = (gg.a * vxi - gg.b * veta) + (gg.c * ueta - gg.d * uxi);
-                if constexpr (SPH) {
-                    g11 -= gg.js * vq;
-                    g12 += gg.js * uq;
-                }
-                g12 *= 0.5;
-                double bd[8];
-                dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, bd);
-#pragma unroll
-                for (int k = 0; k < S; ++k) {
-                    if (bd[k] == 0.0)
-                        continue;
-                    t11[k] += bd[k] * g11;
-                    t12[k] += bd[k] * g12;
-                    t22[k] += bd[k] * g22;
-                }
-            }
-        }
-        const uint32_t offM = after(after(after(off, t11[S - 1]), t12[S - 1]), t22[S - 1]);
-        inverseMassApply<CG>([&](int k) { return ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM); }, t11, t12,
-            t22, e11, e12, e22);
-    }
+    strainGeom<CG, SPH>(a, geo, e, e11, e12, e22, [&](const double* g1, int k, uint32_t offM) {
+        return ldAt(g1 + static_cast<size_t>(L::MI + k) * ne, offM);
+    });
     if (__builtin_amdgcn_ballot_w64(!a.ocean[e]) != 0) { // a wave with land: uniform branch
         const bool land = !a.ocean[e];
 #pragma unroll
@@ -3368,8 +3317,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         const ElemArgs a = elemArgs();
         if (mesh.uniform)
             k_strain<CG, true><<<blocksFor(ne), kBlock, 0, s>>>(a);
+        else if (mesh.spherical)
+            k_strain_geom<CG, true><<<blocksFor(ne), kBlock, 0, s>>>(a, geo.p);
         else
-            k_strain<CG, false><<<blocksFor(ne), kBlock, 0, s>>>(a);
+            k_strain_geom<CG, false><<<blocksFor(ne), kBlock, 0, s>>>(a, geo.p);
         checkLaunch("strain");
     }
     void runPasses(int n)
