@@ -70,6 +70,42 @@ template <int DG> __global__ void k_transport_ops(MeshView m, int nOps, double* 
                 = m.spherical ? Mi[i][j] / kEarthRadius : Mi[i][j];
 }
 
+// Transport operators of a general mesh from its geometry, SoA [k][ne]: the bilinear map
+// x(xi, eta) = x0 + X1 xi + X2 eta + X3 xi eta (and y), and the upper triangle of the
+// inverse (metric) DG mass matrix (/ R on the sphere). The cell terms factor through the map:
+//   sum_q (cellX_i vx + cellY_i vy) phi = sum_q psi_xi,i (y_eta vx - x_eta vy) w phi
+//                                            + psi_eta,i (x_xi vy - y_xi vx) w phi,
+// so 6 + D (D + 1) / 2 values (27 for D = 6, 42 for D = 8) replace 2 D Q + D^2 (144, 208).
+template <int DG> struct TrGeo {
+    static constexpr int X1 = 0, X2 = 1, X3 = 2, Y1 = 3, Y2 = 4, Y3 = 5, MI = 6, K = 6 + DG * (DG + 1) / 2;
+    __host__ __device__ static constexpr int mi(int i, int j)
+    {
+        return i <= j ? MI + i * DG - i * (i - 1) / 2 + (j - i) : mi(j, i);
+    }
+};
+template <int DG> __global__ void k_transport_geom(MeshView m, int ne, double* ops)
+{
+    using L = TrGeo<DG>;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    auto put = [&](int k, double v) { ops[static_cast<size_t>(k) * ne + e] = v; };
+    double c[4][2]; // LL, LR, UL, UR
+    elementCoords(m, e, c);
+    put(L::X1, c[1][0] - c[0][0]);
+    put(L::X2, c[2][0] - c[0][0]);
+    put(L::X3, (c[0][0] - c[1][0]) - (c[2][0] - c[3][0]));
+    put(L::Y1, c[1][1] - c[0][1]);
+    put(L::Y2, c[2][1] - c[0][1]);
+    put(L::Y3, (c[0][1] - c[1][1]) - (c[2][1] - c[3][1]));
+    double M[DG][DG], Mi[DG][DG];
+    metricMassMatrix<DG>(m, c, M);
+    invert<DG>(M, Mi);
+    for (int i = 0; i < DG; ++i)
+        for (int j = i; j < DG; ++j)
+            put(L::mi(i, j), m.spherical ? Mi[i][j] / kEarthRadius : Mi[i][j]);
+}
+
 // ---------------------------------------------------------------- normal velocities
 // reinitnormalvelocity (DGTransport.cpp:158-252), one thread per edge: the average of
 // the two adjacent ocean cells' traces along the scaled normal, times 2^k where k
@@ -269,8 +305,12 @@ template <int DG, bool UNI> __device__ __forceinline__ void finishCell(const TrA
     for (int i = 0; i < DG; ++i) {
         double k = 0;
 #pragma unroll
-        for (int j = 0; j < DG; ++j)
-            k += op(2 * DG * Q + i * DG + j, e) * up[j];
+        for (int j = 0; j < DG; ++j) {
+            if constexpr (UNI)
+                k += op(2 * DG * Q + i * DG + j, e) * up[j];
+            else
+                k += a.ops[static_cast<size_t>(TrGeo<DG>::mi(i, j)) * ne + e] * up[j];
+        }
         const size_t o = i * ne + e;
         double r;
         switch (a.mode) {
@@ -323,13 +363,30 @@ template <int DG, bool UNI, bool FINISH> __global__ __launch_bounds__(kBlock) vo
                 vyg[q] = s1;
                 pg[q] = s2;
             }
+            if constexpr (UNI) {
 #pragma unroll
-            for (int i = 0; i < DG; ++i) {
-                double s = 0;
+                for (int i = 0; i < DG; ++i) {
+                    double s = 0;
 #pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    s += a.dt * (op(i * Q + q, e) * vxg[q] + op(DG * Q + i * Q + q, e) * vyg[q]) * pg[q];
-                up[i] += s;
+                    for (int q = 0; q < Q; ++q)
+                        s += a.dt * (op(i * Q + q, e) * vxg[q] + op(DG * Q + i * Q + q, e) * vyg[q]) * pg[q];
+                    up[i] += s;
+                }
+            } else {
+                using L = TrGeo<DG>;
+                const double X1 = a.ops[L::X1 * ne + e], X2 = a.ops[L::X2 * ne + e], X3 = a.ops[L::X3 * ne + e];
+                const double Y1 = a.ops[L::Y1 * ne + e], Y2 = a.ops[L::Y2 * ne + e], Y3 = a.ops[L::Y3 * ne + e];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const double xi = gauss_point(G, q % G), eta = gauss_point(G, q / G);
+                    const double w = gauss_weight(G, q % G) * gauss_weight(G, q / G) * a.dt;
+                    const double xxi = X1 + X3 * eta, xeta = X2 + X3 * xi, yxi = Y1 + Y3 * eta, yeta = Y2 + Y3 * xi;
+                    const double fxi = (yeta * vxg[q] - xeta * vyg[q]) * (w * pg[q]);
+                    const double feta = (xxi * vyg[q] - yxi * vxg[q]) * (w * pg[q]);
+#pragma unroll
+                    for (int i = 0; i < DG; ++i)
+                        up[i] += Psi<DG, G>::x[i][q] * fxi + Psi<DG, G>::y[i][q] * feta;
+                }
             }
         }
     }
@@ -661,9 +718,13 @@ template <int DG> struct TransportImpl final : TransportBase {
         const size_t ne = mesh.ne;
         nXe = mesh.nx * (mesh.ny + 1);
         nYe = (mesh.nx + 1) * mesh.ny;
-        const int nOps = mesh.uniform ? 1 : static_cast<int>(ne);
-        ops.alloc(static_cast<size_t>(2 * DG * Q + DG * DG) * nOps);
-        k_transport_ops<DG><<<blocksFor(nOps), kBlock, 0, s>>>(view(mesh), nOps, ops.p);
+        if (mesh.uniform) {
+            ops.alloc(static_cast<size_t>(2 * DG * Q + DG * DG));
+            k_transport_ops<DG><<<1, kBlock, 0, s>>>(view(mesh), 1, ops.p);
+        } else {
+            ops.alloc(static_cast<size_t>(TrGeo<DG>::K) * ne);
+            k_transport_geom<DG><<<blocksFor(ne), kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), ops.p);
+        }
         checkLaunch("transport_ops");
         for (auto* b : { &vx, &vy, &tmpA, &tmpB, &kbuf, &raw })
             b->alloc(ne * DG);
