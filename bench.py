@@ -148,11 +148,12 @@ def spherical_leg(nx, steps, warmup, dt=120.0, seed=0, traffic_path=None):
 def brittle_byte_model(nx, ny, cg=2, dg=6):
     """Algorithmic HBM bytes of one one-pass brittle substep: stress read + written (3 S
     coefficients each), damage read + written, h and A read (D coefficients each) per
-    element; per node u, v read + written, the 9 forcing and geometry values (H, A, ocean
-    and wind velocity, SSH gradient, lumped mass) read, the averaged velocity read + written."""
+    element; per node u, v read + written, the 7 terms fixed over the substeps (ocean
+    velocity, dt / (rho H) and its ratio to the lumped mass, the wind and SSH forcing, A F_ocean)
+    read, the averaged velocity read + written."""
     S = 8 if cg == 2 else 3
     ne, nn = nx * ny, (cg * nx + 1) * (cg * ny + 1)
-    return 8 * (6 * S + 4 * dg) * ne + 8 * 17 * nn
+    return 8 * (6 * S + 4 * dg) * ne + 8 * 15 * nn
 
 
 def brittle_general_byte_model(nx, ny, cg=2, dg=6):

@@ -2405,9 +2405,14 @@ __global__ __launch_bounds__(kBlock) void k_bbm_element(ElemArgs a, BrittleArgs 
 // BrittleCGDynamicsKernel::updateMomentum (BrittleCGDynamicsKernel.hpp:200-253) for one
 // node: the semi-implicit substep with the Coriolis term and the ocean drag turned by the
 // turning angle, solved as the reference writes it (its v carries +beta (gradX + tauX)),
-// and the step-averaged velocity accumulated before the Dirichlet zero.
+// and the step-averaged velocity accumulated before the Dirichlet zero. The terms fixed
+// over the substeps of a step come from k_brittle_coeffs (a.ocn = (uo, vo), a.cf = (dtm,
+// dtm / ML) with dtm = dt / (rho_ice H), a.qf = dtm (dA ua - rho_ice H g gx, dA va -
+// rho_ice H g gy) with dA = A F_atm |ua|, af = A F_ocean), so that
+//   dtm (gradX + tauX) = dtm / ML dSx + qf.x + dtm cP (uo cos - vo sin), cP = af |uo - u|.
 struct BrittleIn {
-    double u, v, H, A, uo, vo, ua, va, gx, gy, ML, au, av;
+    double u, v, au, av, af;
+    double2 ocn, cf, qf;
 };
 struct BrittleNode {
     using In = BrittleIn;
@@ -2415,19 +2420,15 @@ struct BrittleNode {
     double ct, st, nsteps;
     double* avgU;
     double* avgV;
+    const double* af;
     __device__ __forceinline__ void load(const NodeArgs& a, int n, In& in) const
     {
         in.u = a.u[n];
         in.v = a.v[n];
-        in.H = a.H[n];
-        in.A = a.A[n];
-        in.uo = a.uo[n];
-        in.vo = a.vo[n];
-        in.ua = a.ua[n];
-        in.va = a.va[n];
-        in.gx = a.gx[n];
-        in.gy = a.gy[n];
-        in.ML = a.ML[n];
+        in.ocn = a.ocn[n];
+        in.cf = a.cf[n];
+        in.qf = a.qf[n];
+        in.af = af[n];
         in.au = avgU[n];
         in.av = avgV[n];
     }
@@ -2435,18 +2436,15 @@ struct BrittleNode {
     __device__ __forceinline__ void update(const NodeArgs& a, const In& in, int n, bool dir, double dSx, double dSy,
         double& un, double& vn) const
     {
-        const double dtm = a.dt / (a.rho_ice * in.H);
-        const double cP = in.A * a.F_ocean * hypot(in.uo - in.u, in.vo - in.v);
-        const double al = 1.0 + dtm * (cP * ct);
-        const double be = a.dt * a.fc + dtm * cP * st;
+        const double dtm = in.cf.x, dtmML = in.cf.y;
+        const double dcP = dtm * (in.af * hypot(in.ocn.x - in.u, in.ocn.y - in.v));
+        const double al = 1.0 + dcP * ct;
+        const double be = a.dt * a.fc + dcP * st;
         const double rD = 1.0 / (al * al + be * be);
-        const double dA = in.A * a.F_atm * hypot(in.ua, in.va);
-        const double tX = dA * in.ua + cP * (in.uo * ct - in.vo * st);
-        const double tY = dA * in.va + cP * (in.vo * ct + in.uo * st);
-        const double gX = dSx / in.ML - a.rho_ice * in.H * a.gravity * in.gx;
-        const double gY = dSy / in.ML - a.rho_ice * in.H * a.gravity * in.gy;
-        un = (al * in.u + be * in.v + dtm * (al * (gX + tX) + be * (gY + tY))) * rD;
-        vn = (al * in.v - be * in.u + dtm * (al * (gY + tY) + be * (gX + tX))) * rD;
+        const double GX = dtmML * dSx + in.qf.x + dcP * (in.ocn.x * ct - in.ocn.y * st); // dtm (gradX + tauX)
+        const double GY = dtmML * dSy + in.qf.y + dcP * (in.ocn.y * ct + in.ocn.x * st); // dtm (gradY + tauY)
+        un = (al * in.u + be * in.v + (al * GX + be * GY)) * rD;
+        vn = (al * in.v - be * in.u + (al * GY + be * GX)) * rD;
         avgU[n] = in.au + un / nsteps;
         avgV[n] = in.av + vn / nsteps;
         if (dir) {
@@ -2455,6 +2453,22 @@ struct BrittleNode {
         }
     }
 };
+// The terms of BrittleNode fixed over the substeps of one step (a.dt = the substep).
+__global__ __launch_bounds__(kBlock) void k_brittle_coeffs(NodeArgs a, double2* ocn, double2* cf, double2* qf,
+    double* af)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nn)
+        return;
+    const double H = a.H[n], A = a.A[n], ua = a.ua[n], va = a.va[n];
+    const double dtm = a.dt / (a.rho_ice * H);
+    const double dA = A * a.F_atm * hypot(ua, va);
+    const double rhoHg = a.rho_ice * H * a.gravity;
+    ocn[n] = make_double2(a.uo[n], a.vo[n]);
+    cf[n] = make_double2(dtm, dtm / a.ML[n]);
+    qf[n] = make_double2(dtm * (dA * ua - rhoHg * a.gx[n]), dtm * (dA * va - rhoHg * a.gy[n]));
+    af[n] = A * a.F_ocean;
+}
 // Split pass: every node, the Dirichlet zero left to k_boundaries.
 __global__ __launch_bounds__(kBlock) void k_brittle_momentum(NodeArgs a, BrittleNode pol)
 {
@@ -2748,6 +2762,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<uint8_t> dirNode;
     DBuf<double2> fzOcn, fzCf, fzQ; // frozen momentum coefficients (k_node_coeffs)
     DBuf<double> strength;          // ice strength at the Gauss points
+    DBuf<double> fzAF;              // brittle: A F_ocean per node
     DBuf<float> sigF, strengthF;    // FP32 variant: stress and strength
     DBuf<float> sigFAlt;            // ping-pong partner of sigF (fused subcycle)
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
@@ -2888,6 +2903,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         deviceFill(damage.p, ne, 1.0, s);
         avgU.alloc(nn);
         avgV.alloc(nn);
+        fzAF.alloc(nn);
         deviceFill(avgU.p, nn, 0.0, s);
         deviceFill(avgV.p, nn, 0.0, s);
         // sqrt of the quadrilateral area (ParametricMesh.hpp:276-299) of element 0, shared
@@ -3417,7 +3433,14 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     }
     BrittleNode brittleNode() const
     {
-        return BrittleNode { turnCos(), turnSin(), static_cast<double>(p.nsteps), avgU.p, avgV.p };
+        return BrittleNode { turnCos(), turnSin(), static_cast<double>(p.nsteps), avgU.p, avgV.p, fzAF.p };
+    }
+    // The node terms fixed over the substeps of a step (from the current fields and deltaT).
+    void brittleFrozenAsync()
+    {
+        Timed t(prof, s, "frozen_terms");
+        k_brittle_coeffs<<<blocksFor(nn), kBlock, 0, s>>>(nodeArgs(), fzOcn.p, fzCf.p, fzQ.p, fzAF.p);
+        checkLaunch("brittle_coeffs");
     }
     void brittleMomentumAsync()
     {
@@ -3465,6 +3488,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // per-element operators, sizes and damage projections.
     void brittleSubstepsAsync(int n)
     {
+        if (n > 0)
+            brittleFrozenAsync();
         for (int k = 0; k < n; ++k) {
             if (!mesh.hasPeriodic) {
                 if (k == n - 1)
@@ -3570,10 +3595,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     {
         need();
         require(deltaT > 0, "begin_iteration must set dt first", NSDG_ERR_STATE);
-        if (brittle)
+        if (brittle) {
+            brittleFrozenAsync();
             brittleMomentumAsync();
-        else
+        } else {
             momentumAsync();
+        }
         synchronize();
     }
     void boundaries() override
