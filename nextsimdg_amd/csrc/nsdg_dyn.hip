@@ -2112,9 +2112,12 @@ __device__ __forceinline__ void bbmPoint(const BrittleArgs& b, double hg, double
     const double sn = 0.5 * (y11 + y22);
     const double tau = sqrt(0.25 * (y11 - y22) * (y11 - y22) + y12 * y12);
     const double cohesion = b.cohesion * hg, cstr = b.compr * hg;
-    double dcrit = (tau + b.tanPhi * sn > 0.0) ? cohesion / (tau + b.tanPhi * sn) : 1.0;
-    dcrit = sn < -cstr ? -cstr / sn : dcrit;
-    dcrit = fmin(dcrit, 1.0);
+    // d_crit = cohesion / (tau + tan(phi) sn) if that denominator is positive, else 1; the
+    // compressive branch -cstr / sn replaces it when sn < -cstr: the operands are chosen
+    // first, so one division serves both branches
+    const bool compressive = sn < -cstr, shear = tau + b.tanPhi * sn > 0.0;
+    const double num = compressive ? -cstr : cohesion, den = compressive ? sn : tau + b.tanPhi * sn;
+    const double dcrit = fmin(compressive || shear ? num / den : 1.0, 1.0);
     const double relax = (1.0 - dcrit) * (b.dtOverT * sqrt(E));
     dg -= dg * relax;
     y11 -= y11 * relax;
