@@ -5,10 +5,23 @@
     stress, divergence, momentum, Dirichlet) run through the per-pass entry points;
   * a model step is deterministic: two kernels given the same inputs agree bit for bit
     (every gather is atomic-free in a fixed order);
-  * the state stays finite and the Dirichlet nodes exactly zero.
+  * the state stays finite and the Dirichlet nodes exactly zero;
+  * the DG advection conserves the ice volume (Tools::MeanValue of h) when no flux
+    leaves through the coast: edges between two ocean cells pass equal and opposite
+    fluxes, edges that touch land carry none (DGTransport.cpp:360-363), and the outflow
+    term of the Dirichlet edges along the coast (DGTransport.cpp:306-351, 484-505)
+    vanishes when the velocity is zero near the coast; h never reaches its lower limit,
+    so the volume changes by rounding only;
+  * the same one-pass and determinism properties for the brittle (BBM) substep.
 
-Tolerance of the first: 1e-12 relative L2 per field after 3 subcycles (the two paths
-associate the same sums differently).
+Tolerances: 1e-12 relative L2 per field after 3 mEVP subcycles (the two paths associate
+the same sums differently). For 7 BBM substeps, 1e-12 or 10x the spread of the split path
+itself when its velocity and stress are perturbed by one ulp before every substep (the
+two paths round differently in every element at every substep, not once), whichever is
+larger: at 1024^2 (500 m cells) the elastic substep amplifies rounding to ~1e-11 in the
+strain within 7 substeps.
+The volume after 5 advection steps: 1e-12 relative (about 1e-16 per element and step,
+summed over up to 4.2e6 elements).
 """
 import numpy as np
 import pytest
@@ -22,12 +35,12 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def _kernel(nx, seed=0):
+def _kernel(nx, seed=0, brittle=False):
     import nextsimdg_amd as nd
     from nextsimdg_amd import inputs
 
     s = inputs.benchmark_setup(nx, nx, seed)
-    k = nd.MEVPDynamicsKernel(2, 6)
+    k = nd.BBMDynamicsKernel(2, 6) if brittle else nd.MEVPDynamicsKernel(2, 6)
     k.initialise(s["coords"], False, s["mask"], nx, nx)
     for name, key in (("hice", "h"), ("cice", "A"), ("uocean", "uocean"), ("vocean", "vocean")):
         k.setData(name, s[key], 1)
@@ -83,3 +96,92 @@ def test_model_step_is_deterministic_and_finite_at_size(nx):
         assert np.all(f[lo, lo:hi + 1] == 0.0) and np.all(f[hi, lo:hi + 1] == 0.0), n
         assert np.all(f[lo:hi + 1, lo] == 0.0) and np.all(f[lo:hi + 1, hi] == 0.0), n
         assert np.abs(f[lo + 1:hi, lo + 1:hi]).max() > 1e-3, n
+
+
+@pytest.mark.parametrize("nx", [512, 2048])
+def test_advection_conserves_ice_volume_at_size(nx):
+    _say(f"volume conservation at {nx}^2")
+    k = _kernel(nx, seed=5)
+    k.update(120.0)  # a developed velocity field under the cyclone
+    # the same field set to zero within 4 elements of the land ring (8 CG2 nodes), so the
+    # Dirichlet edges along the coast see no outflow
+    row = 2 * nx + 1
+    idx = np.arange(row)
+    inner = (idx >= 10) & (idx <= row - 11)
+    window = np.outer(inner, inner).ravel()
+    for n in ("u", "v"):
+        k.set_cg(n, k.get_cg(n) * window)
+    v0, _ = k.mass()
+    for _ in range(5):
+        k.advectionAndLimits(120.0)
+    v1, _ = k.mass()
+    h = k.getDGData("hice", 6)
+    _say(f"  volume {v0:.15e} -> {v1:.15e} (rel {abs(v1 - v0) / v0:.1e}), min h mean {h[:, 0].min():.3e}")
+    assert h[:, 0].min() > 0.0  # the lower limit never acted on a mean
+    assert abs(v1 - v0) <= 1e-12 * abs(v0)
+
+
+@pytest.mark.parametrize("nx", [512, 1024])
+def test_bbm_one_pass_substeps_equal_split_passes_at_size(nx):
+    _say(f"BBM fused vs split at {nx}^2")
+    a, b, c = (_kernel(nx, brittle=True) for _ in range(3))
+    for k in (a, b, c):
+        k.update(120.0)  # velocity, stress and damage from one step of 100 substeps
+        k.prepareIteration()
+        k.begin_iteration(1.2)
+    rng = np.random.default_rng(2)
+
+    def ulp(x):
+        return x * (1.0 + 2.0 ** -52 * rng.choice([-1.0, 1.0], x.shape))
+
+    d0 = b.getDGData("damage", 6)
+    a.subcycles(7)
+    for k in (b, c):
+        for _ in range(7):
+            if k is c:
+                for n in ("u", "v"):
+                    k.set_cg(n, ulp(k.get_cg(n)))
+                for n in ("s11", "s12", "s22"):
+                    k.setData(n, ulp(k.getDGData(n, 8)), 8)
+            k.projectVelocityToStrain()
+            k.stressUpdateHighOrder(1)
+            k.stressDivergence()
+            k.updateMomentum()
+            k.applyBoundaries()
+    a.synchronize()
+
+    def diffs(x, y):
+        d = {n: rel_l2(x.get_cg(n), y.get_cg(n)) for n in ("u", "v", "avgU", "avgV")}
+        d.update({n: rel_l2(x.getDGData(n, 8), y.getDGData(n, 8)) for n in ("s11", "s12", "s22", "e11", "e12", "e22")})
+        d["damage"] = rel_l2(x.getDGData("damage", 6), y.getDGData("damage", 6))
+        return d
+
+    errs, spread = diffs(a, b), diffs(c, b)
+    failed = int((b.getDGData("damage", 6)[:, 0] < d0[:, 0] - 1e-12).sum())
+    _say("  fused-split " + " ".join(f"{n}={e:.1e}" for n, e in errs.items()) + f"; {failed} elements damaged further")
+    _say("  per-substep 1-ulp spread " + " ".join(f"{n}={e:.1e}" for n, e in spread.items()))
+    assert np.abs(a.get_cg("u")).max() > 1e-3
+    for n, e in errs.items():
+        assert np.isfinite(e) and e < max(1e-12, 10.0 * spread[n]), (n, e, spread[n])
+
+
+def test_bbm_step_is_deterministic_and_bounded_at_size():
+    nx = 512
+    _say(f"BBM determinism at {nx}^2")
+    a, b = _kernel(nx, seed=4, brittle=True), _kernel(nx, seed=4, brittle=True)
+    for k in (a, b):
+        k.update(120.0)
+        k.synchronize()
+    for n in ("u", "v", "avgU", "avgV"):
+        fa = a.get_cg(n)
+        assert np.array_equal(fa, b.get_cg(n)), n
+        assert np.isfinite(fa).all(), n
+    for n, nc in (("hice", 6), ("cice", 6), ("damage", 6), ("s11", 8), ("s12", 8), ("s22", 8)):
+        assert np.array_equal(a.getDGData(n, nc), b.getDGData(n, nc)), n
+    # the damage mean stays in (0, 1] and the Dirichlet nodes are zero
+    d = a.getDGData("damage", 6)[:, 0]
+    assert d.min() > 0.0 and d.max() <= 1.0 + 1e-12
+    row = 2 * nx + 1
+    f = a.get_cg("u").reshape(row, row)
+    lo, hi = 2, 2 * nx - 2
+    assert np.all(f[lo, lo:hi + 1] == 0.0) and np.all(f[lo:hi + 1, lo] == 0.0)
