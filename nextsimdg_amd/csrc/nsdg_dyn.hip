@@ -2807,6 +2807,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         ne = mesh.ne;
         nn = static_cast<size_t>(CG * mesh.nx + 1) * (CG * mesh.ny + 1);
         nn1 = static_cast<size_t>(mesh.nx + 1) * (mesh.ny + 1);
+        // the kernels index CG nodes with 32-bit ints (the mesh checks elements and vertices)
+        require(nn < (size_t(1) << 31), "too many CG nodes for 32-bit node indices", NSDG_ERR_ARG);
         tr = makeTransport(DG, mesh, s, prof);
         tr->setScheme(2); // DynamicsKernel.hpp:61
         cg2dg.build(CG, DG, mesh, s);
