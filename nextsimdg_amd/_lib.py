@@ -81,6 +81,7 @@ SIGNATURES = {
     "nsdg_dyn_apply_boundaries": (_i, [_vp]),
     "nsdg_dyn_subcycles": (_i, [_vp, _i]),
     "nsdg_dyn_set_precision": (_i, [_vp, _i]),
+    "nsdg_dyn_set_pairing": (_i, [_vp, _i]),
     "nsdg_dyn_benchmark_wind": (_i, [_vp, _d, _d, _d, _d, _d]),
     "nsdg_dyn_reset_dg0": (_i, [_vp, _s]),
     "nsdg_dyn_mass": (_i, [_vp, _dp, _dp]),

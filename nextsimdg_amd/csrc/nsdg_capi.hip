@@ -407,6 +407,10 @@ int nsdg_dyn_set_precision(nsdg_dyn* d, int precision)
 {
     return guard([&] { nonNull(d, "dyn")->k->setPrecision(precision); });
 }
+int nsdg_dyn_set_pairing(nsdg_dyn* d, int mode)
+{
+    return guard([&] { nonNull(d, "dyn")->k->setPairing(mode); });
+}
 int nsdg_dyn_benchmark_wind(nsdg_dyn* d, double L, double t, double xoff, double yoff, double vmax)
 {
     return guard([&] { nonNull(d, "dyn")->k->benchmarkWind(L, t, xoff, yoff, vmax); });

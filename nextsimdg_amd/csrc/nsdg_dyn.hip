@@ -824,27 +824,21 @@ template <> __device__ __forceinline__ const float* strengthOf<float>(const Elem
 // which the gradient needs, are formed exactly in FP64 before rounding to FP32).
 // VP: sigma <- Pi sigma_VP(eps(u)) outright (no relaxation, the old stress is not read);
 // used for the momentum residual and for the stress of a predicted velocity.
-template <int CG, int DG, class R, class C, bool VP>
-__device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3)
+// vel(lu, lv) gives the element's node velocities by node row (ocean elements only);
+// old(c, s) the previous stress, component c (11, 12, 22), coefficient s.
+template <int CG, int DG, class R, class C, bool VP, class VEL, class OLD>
+__device__ __forceinline__ void elementAffineCore(
+    const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3, const VEL& vel, const OLD& old)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
     const size_t ne = a.ne;
     const bool oc = a.ocean[e] != 0;
-    R* const sig = stressOf<R>(a);
     const R* const Pg = strengthOf<R>(a);
     C buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
     if (oc) {
-        const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
-        const int cgi = CG * row * iy + CG * ix;
         double lu[N], lv[N];
-#pragma unroll
-        for (int r = 0; r < K; ++r)
-#pragma unroll
-            for (int c = 0; c < K; ++c) {
-                lu[c + K * r] = a.u[cgi + c + r * row];
-                lv[c + K * r] = a.v[cgi + c + r * row];
-            }
+        vel(lu, lv);
         const double cu = sizeof(C) == 8 ? 0.0 : lu[N / 2], cv = sizeof(C) == 8 ? 0.0 : lv[N / 2];
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -872,7 +866,7 @@ __device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, A
             acc2[s].init(0);
             acc3[s].init(0);
         } else {
-            const C o1 = sig[(0 * S + s) * ne + e], o2 = sig[(1 * S + s) * ne + e], o3 = sig[(2 * S + s) * ne + e];
+            const C o1 = old(0, s), o2 = old(1, s), o3 = old(2, s);
             const C mInvAlpha = -static_cast<C>(a.invAlpha);
             acc1[s].init(o1);
             acc2[s].init(o2);
@@ -955,11 +949,33 @@ __device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, A
     if constexpr (sizeof(C) == 8 && !VP) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            acc1[s].s = sig[(0 * S + s) * ne + e] * a.relax + acc1[s].s;
-            acc2[s].s = sig[(1 * S + s) * ne + e] * a.relax + acc2[s].s;
-            acc3[s].s = sig[(2 * S + s) * ne + e] * a.relax + acc3[s].s;
+            acc1[s].s = old(0, s) * a.relax + acc1[s].s;
+            acc2[s].s = old(1, s) * a.relax + acc2[s].s;
+            acc3[s].s = old(2, s) * a.relax + acc3[s].s;
         }
     }
+}
+// The element update from the state in HBM (velocity a.u, a.v; stress of precision R).
+template <int CG, int DG, class R, class C, bool VP>
+__device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3)
+{
+    constexpr int S = MomLayout<CG>::S, K = CG + 1;
+    const size_t ne = a.ne;
+    const R* const sig = stressOf<R>(a);
+    elementAffineCore<CG, DG, R, C, VP>(
+        a, e, acc1, acc2, acc3,
+        [&](double* lu, double* lv) {
+            const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
+            const int cgi = CG * row * iy + CG * ix;
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    lu[c + K * r] = a.u[cgi + c + r * row];
+                    lv[c + K * r] = a.v[cgi + c + r * row];
+                }
+        },
+        [&](int c, int k) -> C { return sig[(c * S + k) * ne + e]; });
 }
 template <int CG, int DG, class R, class C = R, bool VP = false>
 __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
@@ -1293,7 +1309,7 @@ static_assert(FusedTile<2>::HX * FusedTile<2>::HY == kBlock, "fused tile must gi
 struct MevpNode {
     using In = NodeFz;
     static constexpr bool kAllNodes = false;
-    __device__ __forceinline__ void load(const NodeArgs& a, int n, In& in) const { loadFrozen(a, n, in); }
+    __device__ __forceinline__ void load(const NodeArgs& a, int n, int, int, In& in) const { loadFrozen(a, n, in); }
     __device__ __forceinline__ void update(const NodeArgs& a, const In& in, int, bool, double dx, double dy,
         double& un, double& vn) const
     {
@@ -1329,7 +1345,7 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const G& contr
                 const int n = nix + row * (ny0 + r);
                 dir[b] = live[b] ? a.dirNode[n] != 0 : true;
                 if (live[b] && (P::kAllNodes || !dir[b]))
-                    pol.load(a, n, in[b]);
+                    pol.load(a, n, nix, ny0 + r, in[b]);
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -1422,6 +1438,196 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
     __syncthreads();
     nodeTileUpdate<CG, T>(
         a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
+}
+
+// Two mEVP subcycles in one pass over a uniform mesh (FP64), the state read from and
+// written to HBM once per two subcycles. A tile owns TX x TY = 29 x 5 elements.
+// - Sub-step 1 updates the tile's elements with a two-element left/bottom and a
+//   one-element right/top halo, 32 x 8, one per thread, from sigma^k, u^k, v^k in HBM;
+//   each thread keeps its sigma^(k+1) in registers, and the block updates the CG nodes
+//   of the inner 30 x 6 elements (61 x 13 nodes for CG2) into LDS.
+// - Sub-step 2 is the one-pass subcycle of those inner elements (the tile and its
+//   one-element left/bottom halo) from the registers and LDS: it stores sigma^(k+2) of the
+//   owned elements and u^(k+2), v^(k+2) of the owned nodes.
+// Sub-step 2 parks sigma^(k+1) in LDS (the sub-step 1 contributions are consumed by then)
+// while its strain and stress sums run, which keeps both sub-steps at the register count
+// of k_mevp_fused. Every value is computed by the arithmetic of k_mevp_fused, in the same
+// order, from the same operands (a halo value identically in every tile that computes
+// it), so a pass gives the bits of two k_mevp_fused passes.
+template <int CG> struct PairTile {
+    static constexpr int TX = 29, TY = 5;
+    static constexpr int W1 = TX + 3, H1 = TY + 3;          // sub-step 1 elements
+    static constexpr int W2 = TX + 1, H2 = TY + 1;          // sub-step 2 elements
+    static constexpr int NW = CG * W2 + 1, NH = CG * H2 + 1; // the CG nodes of those
+    static constexpr int N = cgdofs(CG);
+    using Inner = TileShape<CG, TX, TY>;
+};
+static_assert(PairTile<2>::W1 * PairTile<2>::H1 == kBlock, "pair tile must give one element per thread");
+static_assert(PairTile<2>::Inner::HX == PairTile<2>::W2 && PairTile<2>::Inner::HY == PairTile<2>::H2,
+    "sub-step 2 is the one-pass tile of the owned elements");
+// mEVP node update (as MevpNode) with the velocity entering it from the LDS node window of
+// k_mevp_pair (origin x0, y0 in global node coordinates, width nw).
+struct MevpNodeWindow {
+    using In = NodeFz;
+    static constexpr bool kAllNodes = false;
+    const double* uS;
+    const double* vS;
+    int x0, y0, nw;
+    __device__ __forceinline__ void load(const NodeArgs& a, int n, int nix, int niy, In& in) const
+    {
+        const int l = (nix - x0) + nw * (niy - y0);
+        in.u = uS[l];
+        in.v = vS[l];
+        in.ocn = a.ocn[n];
+        in.cf = a.cf[n];
+        in.qf = a.qf[n];
+    }
+    __device__ __forceinline__ void update(const NodeArgs& a, const In& in, int, bool, double dx, double dy,
+        double& un, double& vn) const
+    {
+        momentumFrozen(a, in, dx, dy, un, vn);
+    }
+};
+template <int CG, int DG>
+__global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
+{
+    using T = PairTile<CG>;
+    constexpr int N = T::N, S = cg2dgstress(CG), K = CG + 1;
+    constexpr int NWH = T::NW * T::NH, NB = (NWH + kBlock - 1) / kBlock;
+    __shared__ double contrib[T::W1 * T::H1][N][2];
+    __shared__ double uS[NWH], vS[NWH];
+    const NodeArgs& a = f.n;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+    const int x0 = CG * (ex0 - 1), y0 = CG * (ey0 - 1); // node window origin
+    const int row = CG * a.nx + 1;
+    const int lx = threadIdx.x % T::W1, ly = threadIdx.x / T::W1;
+    const int ex = ex0 - 2 + lx, ey = ey0 - 2 + ly;
+    const bool inDomain = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
+    const int e = inDomain ? ex + a.nx * ey : 0;
+    double s11[S], s12[S], s22[S]; // sigma^(k+1) of this thread's element
+    {
+        double tx[N], ty[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            tx[j] = ty[j] = 0.0;
+        if (inDomain) {
+            Acc<double> acc1[S], acc2[S], acc3[S];
+            elementAffineCompute<CG, DG, double, double, false>(f.e, e, acc1, acc2, acc3);
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                s11[k] = acc1[k].get();
+                s12[k] = acc2[k].get();
+                s22[k] = acc3[k].get();
+            }
+            if (a.ocean[e])
+                divergenceAffineFrom<CG, double>(a, s11, s12, s22, tx, ty);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            contrib[threadIdx.x][j][0] = tx[j];
+            contrib[threadIdx.x][j][1] = ty[j];
+        }
+    }
+    __syncthreads();
+    // sub-step 1, the nodes of the inner elements (within the domain) into LDS
+    {
+        NodeFz in[NB];
+        bool live[NB], dir[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int i = threadIdx.x + b * kBlock;
+            const int nix = x0 + i % T::NW, niy = y0 + i / T::NW;
+            live[b] = i < NWH && nix >= 0 && nix <= CG * a.nx && niy >= 0 && niy <= CG * a.ny;
+            const int n = nix + row * niy;
+            dir[b] = live[b] ? a.dirNode[n] != 0 : true;
+            if (live[b] && !dir[b])
+                loadFrozen(a, n, in[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (!live[b])
+                continue;
+            const int i = threadIdx.x + b * kBlock;
+            const int nix = x0 + i % T::NW, niy = y0 + i / T::NW;
+            double un = 0.0, vn = 0.0;
+            if (!dir[b]) {
+                double dx = 0.0, dy = 0.0;
+                forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int exa, int eya, int j) {
+                    const int le = (exa - ex0 + 2) + T::W1 * (eya - ey0 + 2);
+                    dx -= contrib[le][j][0];
+                    dy -= contrib[le][j][1];
+                });
+                momentumFrozen(a, in[b], dx, dy, un, vn);
+            }
+            uS[i] = un;
+            vS[i] = vn;
+        }
+    }
+    __syncthreads();
+    // sub-step 2, the inner elements, from sigma^(k+1) parked in LDS and the LDS velocities
+    {
+        constexpr int NI = T::W2 * T::H2;
+        static_assert(3 * S * NI <= T::W1 * T::H1 * N * 2, "sigma^(k+1) must fit the contribution array");
+        double* const sigS = &contrib[0][0][0];
+        const bool inner = lx >= 1 && lx <= T::W2 && ly >= 1 && ly <= T::H2;
+        const int le2 = (lx - 1) + T::W2 * (ly - 1);
+        const bool active = inner && inDomain;
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                sigS[(0 * S + k) * NI + le2] = s11[k];
+                sigS[(1 * S + k) * NI + le2] = s12[k];
+                sigS[(2 * S + k) * NI + le2] = s22[k];
+            }
+        }
+        double tx[N], ty[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            tx[j] = ty[j] = 0.0;
+        if (active) {
+            const int bx = CG * (ex - ex0 + 1), by = CG * (ey - ey0 + 1); // first node in the window
+            Acc<double> acc1[S], acc2[S], acc3[S];
+            elementAffineCore<CG, DG, double, double, false>(
+                f.e, e, acc1, acc2, acc3,
+                [&](double* lu, double* lv) {
+#pragma unroll
+                    for (int r = 0; r < K; ++r)
+#pragma unroll
+                        for (int c = 0; c < K; ++c) {
+                            lu[c + K * r] = uS[bx + c + T::NW * (by + r)];
+                            lv[c + K * r] = vS[bx + c + T::NW * (by + r)];
+                        }
+                },
+                [&](int c, int k) -> double { return sigS[(c * S + k) * NI + le2]; });
+            double n11[S], n12[S], n22[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                n11[k] = acc1[k].get();
+                n12[k] = acc2[k].get();
+                n22[k] = acc3[k].get();
+            }
+            if (lx >= 2 && ly >= 2)
+                storeTensor<S>(f.sigOut, a.ne, e, n11, n12, n22);
+            if (a.ocean[e])
+                divergenceAffineFrom<CG, double>(a, n11, n12, n22, tx, ty);
+        }
+        __syncthreads(); // every parked stress is read before the contributions overwrite it
+        if (inner) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                contrib[le2][j][0] = tx[j];
+                contrib[le2][j][1] = ty[j];
+            }
+        }
+    }
+    __syncthreads();
+    nodeTileUpdate<CG, typename T::Inner, MevpNodeWindow>(
+        a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut,
+        MevpNodeWindow { uS, vS, x0, y0, T::NW });
 }
 
 // The strain coefficients of one element's velocity on a general mesh (CGDynamicsKernel.cpp:
@@ -2424,7 +2630,7 @@ struct BrittleNode {
     double* avgU;
     double* avgV;
     const double* af;
-    __device__ __forceinline__ void load(const NodeArgs& a, int n, In& in) const
+    __device__ __forceinline__ void load(const NodeArgs& a, int n, int, int, In& in) const
     {
         in.u = a.u[n];
         in.v = a.v[n];
@@ -2479,7 +2685,7 @@ __global__ __launch_bounds__(kBlock) void k_brittle_momentum(NodeArgs a, Brittle
     if (n >= a.nn)
         return;
     BrittleIn in;
-    pol.load(a, n, in);
+    pol.load(a, n, 0, 0, in);
     double un, vn;
     pol.update(a, in, n, false, a.dSx[n], a.dSy[n], un, vn);
     a.u[n] = un;
@@ -2757,6 +2963,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     CG2DGOps cg2dg;
     bool ready = false;
     int precision = NSDG_PREC_FP64;
+    int pairing = NSDG_PAIR_AUTO;
     size_t ne = 0, nn = 0, nn1 = 0;
     double deltaT = 0;
     DBuf<double> mops, geo, brittleGeo, lumped, lumped1, h, A, ssh, eps, sig;
@@ -3344,9 +3551,43 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             k_strain_geom<CG, false><<<blocksFor(ne), kBlock, 0, s>>>(a, geo.p);
         checkLaunch("strain");
     }
+    // Two subcycles per pass: uniform FP64 meshes without periodic segments, by default
+    // from NSDG_PAIR_MIN_ELEMENTS up (measured per subcycle against k_mevp_fused: 1.06x the
+    // time at 512^2, where the state largely stays in the memory-side cache; 0.95x at
+    // 640^2, 0.90x at 1024^2, 0.89x at 2048^2).
+    bool pairSubcycles() const
+    {
+        const bool can = fusedSubcycle() && mesh.uniform && precision == NSDG_PREC_FP64;
+        return can && (pairing == NSDG_PAIR_ALWAYS || (pairing == NSDG_PAIR_AUTO && ne >= NSDG_PAIR_MIN_ELEMENTS));
+    }
+    void pairPass()
+    {
+        Timed t(prof, s, "mevp_pair");
+        FusedArgs f {};
+        f.e = elemArgs();
+        f.n = nodeArgs();
+        f.sigOut = sigAlt.p;
+        f.uOut = uAlt.p;
+        f.vOut = vAlt.p;
+        using T = PairTile<CG>;
+        const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+        k_mevp_pair<CG, DG><<<grid, kBlock, 0, s>>>(f);
+        checkLaunch("mevp_pair");
+        swapBuf(sig, sigAlt);
+        swapBuf(u, uAlt);
+        swapBuf(v, vAlt);
+    }
     void runPasses(int n)
     {
-        for (int k = 0; k < n; ++k) {
+        int k = 0;
+        // pairs while a subcycle remains after the pair: the last one runs alone, after the
+        // strain output of the velocity entering it
+        if (pairSubcycles())
+            for (; k + 2 < n; k += 2) {
+                pairPass();
+                subcycleCount += 2;
+            }
+        for (; k < n; ++k) {
             if (k == n - 1)
                 strainAsync();
             if (fusedSubcycle()) {
@@ -3759,6 +4000,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             sigFAlt.alloc(3 * S * ne);
             strengthF.alloc(static_cast<size_t>(L::Q) * ne);
         }
+    }
+    void setPairing(int mode) override
+    {
+        require(mode == NSDG_PAIR_AUTO || mode == NSDG_PAIR_ALWAYS || mode == NSDG_PAIR_NEVER,
+            "pairing must be NSDG_PAIR_AUTO, NSDG_PAIR_ALWAYS or NSDG_PAIR_NEVER");
+        pairing = mode;
     }
     void setPrecision(int pr) override
     {

@@ -26,6 +26,7 @@ struct DynBase {
     virtual void boundaries() = 0;
     virtual void subcycles(int n) = 0;
     virtual void setPrecision(int p) = 0;
+    virtual void setPairing(int mode) = 0;
     virtual void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) = 0;
     virtual void resetDG0(const std::string& name) = 0;
     virtual void mass(double* vol, double* area) = 0;

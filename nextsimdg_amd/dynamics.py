@@ -257,6 +257,10 @@ class MEVPDynamicsKernel:
     def set_precision(self, prec):
         check(self.lib.nsdg_dyn_set_precision(self.h, int(prec)))
 
+    def set_pairing(self, mode):
+        """0 (auto, by mesh size), 1 (two subcycles per pass), 2 (one per pass)."""
+        check(self.lib.nsdg_dyn_set_pairing(self.h, int(mode)))
+
     def benchmark_wind(self, L, t, xoff=0.0, yoff=0.0, vmax=30.0):
         check(self.lib.nsdg_dyn_benchmark_wind(self.h, float(L), float(t), float(xoff), float(yoff), float(vmax)))
 

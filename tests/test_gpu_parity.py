@@ -481,3 +481,58 @@ def test_state_round_trip_and_dg0_reset():
     assert np.all(d[:, 1:] == 0)
     with pytest.raises(Exception):
         g.getDG0Data("no_such_field")
+
+
+@pytest.mark.parametrize("cg,dg", [(2, 6), (1, 3)])
+@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 9), (29, 5), (30, 6), (31, 7), (58, 10), (59, 11), (37, 29)])
+def test_paired_subcycles_give_the_bits_of_single_passes(cg, dg, nx, ny):
+    """Two subcycles per pass (k_mevp_pair, forced on below its size threshold), against
+    one per pass (k_mevp_fused) on the same state: every field bit for bit, on meshes
+    smaller than a pair tile (29 x 5), exact tile multiples and one element past them,
+    with land cells. Nine subcycles: four pairs and the last one alone."""
+    _say(f"pairs vs single passes cg={cg} dg={dg} {nx}x{ny}")
+    nd = _gpu()
+    S = 8 if cg == 2 else 3
+    coords = box_coords(nx, ny, 2000.0)
+    mask = np.ones(nx * ny)
+    if nx * ny >= 6:
+        mask[(ny // 2) * nx + nx // 2] = 0.0
+        mask[0] = 0.0
+    ne, ncg = nx * ny, (cg * nx + 1) * (cg * ny + 1)
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + cg)
+    state = {"hice": np.abs(rng.normal(1.0, 0.3, (ne, dg))) * np.r_[1.0, [0.05] * (dg - 1)],
+             "cice": np.clip(rng.normal(0.9, 0.05, (ne, dg)), 0, 1) * np.r_[1.0, [0.02] * (dg - 1)]}
+    for n in ("s11", "s12", "s22", "e11", "e12", "e22"):
+        state[n] = rng.normal(0, 1e3 if n[0] == "s" else 1e-7, (ne, S))
+    forcing = {n: rng.normal(0, sc, ne) for n, sc in (("uwind", 10), ("vwind", 10), ("uocean", 0.05),
+                                                       ("vocean", 0.05), ("ssh", 0.1))}
+    vel = {n: rng.normal(0, 0.1, ncg) for n in ("u", "v")}
+    ks = []
+    for mode in (1, 2):  # NSDG_PAIR_ALWAYS, NSDG_PAIR_NEVER
+        k = nd.MEVPDynamicsKernel(cg, dg)
+        k.initialise(coords, False, mask, nx, ny)
+        k.set_pairing(mode)
+        for n, v in state.items():
+            k.setData(n, v, v.shape[1])
+        for n, v in forcing.items():
+            k.setData(n, v, 1)
+        for n, v in vel.items():
+            k.set_cg(n, v)
+        k.prepareIteration()
+        k.begin_iteration(120.0)
+        k.subcycles(9)
+        k.synchronize()
+        ks.append(k)
+    a, b = ks
+    assert np.abs(a.get_cg("u")).max() > 1e-3
+    for n in ("u", "v"):
+        assert np.array_equal(a.get_cg(n), b.get_cg(n)), n
+    for n in ("s11", "s12", "s22", "e11", "e12", "e22"):
+        assert np.array_equal(a.getDGData(n, S), b.getDGData(n, S)), n
+
+
+def test_pairing_mode_is_checked():
+    nd = _gpu()
+    k = nd.MEVPDynamicsKernel(2, 6)
+    with pytest.raises(nd.NsdgError):
+        k.set_pairing(3)
