@@ -190,7 +190,7 @@ int nsdg_dyn_set_precision(nsdg_dyn* d, int precision);
 #define NSDG_PAIR_AUTO 0
 #define NSDG_PAIR_ALWAYS 1
 #define NSDG_PAIR_NEVER 2
-#define NSDG_PAIR_MIN_ELEMENTS 393216
+#define NSDG_PAIR_MIN_ELEMENTS 300000
 int nsdg_dyn_set_pairing(nsdg_dyn* d, int mode);
 /* Benchmark cyclone wind (BenchmarkAtmosphere.cpp:29-76) evaluated on the device at the
  * cell lower-left corners x = i L/nx, y = j L/ny for time t (seconds since the start),

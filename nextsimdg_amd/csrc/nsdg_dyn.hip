@@ -1441,21 +1441,21 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
 }
 
 // Two mEVP subcycles in one pass over a uniform mesh (FP64), the state read from and
-// written to HBM once per two subcycles. A tile owns TX x TY = 29 x 5 elements.
+// written to HBM once per two subcycles. A tile owns TX x TY = 13 x 13 elements.
 // - Sub-step 1 updates the tile's elements with a two-element left/bottom and a
-//   one-element right/top halo, 32 x 8, one per thread, from sigma^k, u^k, v^k in HBM;
+//   one-element right/top halo, 16 x 16, one per thread, from sigma^k, u^k, v^k in HBM;
 //   each thread keeps its sigma^(k+1) in registers, and the block updates the CG nodes
-//   of the inner 30 x 6 elements (61 x 13 nodes for CG2) into LDS.
+//   of the inner 14 x 14 elements (29 x 29 nodes for CG2) into LDS.
 // - Sub-step 2 is the one-pass subcycle of those inner elements (the tile and its
 //   one-element left/bottom halo) from the registers and LDS: it stores sigma^(k+2) of the
 //   owned elements and u^(k+2), v^(k+2) of the owned nodes.
 // Sub-step 2 parks sigma^(k+1) in LDS (the sub-step 1 contributions are consumed by then)
-// while its strain and stress sums run, which keeps both sub-steps at the register count
-// of k_mevp_fused. Every value is computed by the arithmetic of k_mevp_fused, in the same
-// order, from the same operands (a halo value identically in every tile that computes
-// it), so a pass gives the bits of two k_mevp_fused passes.
+// while its strain and stress sums run. Every value is computed by the arithmetic of
+// k_mevp_fused, in the same order, from the same operands (a halo value identically in
+// every tile that computes it); the compiler may still fuse a multiply-add differently
+// in the two kernels, which moves a value by an ulp.
 template <int CG> struct PairTile {
-    static constexpr int TX = 29, TY = 5;
+    static constexpr int TX = 13, TY = 13;
     static constexpr int W1 = TX + 3, H1 = TY + 3;          // sub-step 1 elements
     static constexpr int W2 = TX + 1, H2 = TY + 1;          // sub-step 2 elements
     static constexpr int NW = CG * W2 + 1, NH = CG * H2 + 1; // the CG nodes of those
@@ -1494,7 +1494,10 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
     using T = PairTile<CG>;
     constexpr int N = T::N, S = cg2dgstress(CG), K = CG + 1;
     constexpr int NWH = T::NW * T::NH, NB = (NWH + kBlock - 1) / kBlock;
-    __shared__ double contrib[T::W1 * T::H1][N][2];
+    // the sub-step contributions, and sigma^(k+1) parked during sub-step 2, in one array
+    constexpr int CWORDS = T::W1 * T::H1 * N * 2, SWORDS = 3 * S * T::W2 * T::H2;
+    __shared__ double contribRaw[CWORDS > SWORDS ? CWORDS : SWORDS];
+    auto& contrib = *reinterpret_cast<double(*)[T::W1 * T::H1][N][2]>(contribRaw);
     __shared__ double uS[NWH], vS[NWH];
     const NodeArgs& a = f.n;
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
@@ -1571,8 +1574,7 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
     // sub-step 2, the inner elements, from sigma^(k+1) parked in LDS and the LDS velocities
     {
         constexpr int NI = T::W2 * T::H2;
-        static_assert(3 * S * NI <= T::W1 * T::H1 * N * 2, "sigma^(k+1) must fit the contribution array");
-        double* const sigS = &contrib[0][0][0];
+        double* const sigS = contribRaw; // 3 S NI <= SWORDS values
         const bool inner = lx >= 1 && lx <= T::W2 && ly >= 1 && ly <= T::H2;
         const int le2 = (lx - 1) + T::W2 * (ly - 1);
         const bool active = inner && inDomain;
@@ -3552,9 +3554,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         checkLaunch("strain");
     }
     // Two subcycles per pass: uniform FP64 meshes without periodic segments, by default
-    // from NSDG_PAIR_MIN_ELEMENTS up (measured per subcycle against k_mevp_fused: 1.06x the
-    // time at 512^2, where the state largely stays in the memory-side cache; 0.95x at
-    // 640^2, 0.90x at 1024^2, 0.89x at 2048^2).
+    // from NSDG_PAIR_MIN_ELEMENTS up, where it is faster per subcycle than k_mevp_fused
+    // (below, the state largely stays in the memory-side cache).
     bool pairSubcycles() const
     {
         const bool can = fusedSubcycle() && mesh.uniform && precision == NSDG_PREC_FP64;

@@ -484,12 +484,17 @@ def test_state_round_trip_and_dg0_reset():
 
 
 @pytest.mark.parametrize("cg,dg", [(2, 6), (1, 3)])
-@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 9), (29, 5), (30, 6), (31, 7), (58, 10), (59, 11), (37, 29)])
-def test_paired_subcycles_give_the_bits_of_single_passes(cg, dg, nx, ny):
+@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 9), (13, 13), (14, 14), (26, 26), (27, 27), (40, 29), (31, 7)])
+def test_paired_subcycles_equal_single_passes(cg, dg, nx, ny):
     """Two subcycles per pass (k_mevp_pair, forced on below its size threshold), against
-    one per pass (k_mevp_fused) on the same state: every field bit for bit, on meshes
-    smaller than a pair tile (29 x 5), exact tile multiples and one element past them,
-    with land cells. Nine subcycles: four pairs and the last one alone."""
+    one per pass (k_mevp_fused) on the same state, on meshes smaller than a pair tile
+    (13 x 13), exact tile multiples and one element past them, with land cells. Nine
+    subcycles: four pairs and the last one alone.
+
+    The pair kernel runs the arithmetic of the single one in the same order, but the
+    compiler may fuse a multiply-add differently in the two kernels: single values then
+    differ by an ulp (1.4e-16 relative in u measured). Bound: 1e-14 relative (max norm)
+    per field. A second pair run must give the same bits (no race between the sub-steps)."""
     _say(f"pairs vs single passes cg={cg} dg={dg} {nx}x{ny}")
     nd = _gpu()
     S = 8 if cg == 2 else 3
@@ -508,7 +513,7 @@ def test_paired_subcycles_give_the_bits_of_single_passes(cg, dg, nx, ny):
                                                        ("vocean", 0.05), ("ssh", 0.1))}
     vel = {n: rng.normal(0, 0.1, ncg) for n in ("u", "v")}
     ks = []
-    for mode in (1, 2):  # NSDG_PAIR_ALWAYS, NSDG_PAIR_NEVER
+    for mode in (1, 1, 2):  # NSDG_PAIR_ALWAYS twice, NSDG_PAIR_NEVER
         k = nd.MEVPDynamicsKernel(cg, dg)
         k.initialise(coords, False, mask, nx, ny)
         k.set_pairing(mode)
@@ -523,12 +528,19 @@ def test_paired_subcycles_give_the_bits_of_single_passes(cg, dg, nx, ny):
         k.subcycles(9)
         k.synchronize()
         ks.append(k)
-    a, b = ks
+    a, a2, b = ks
     assert np.abs(a.get_cg("u")).max() > 1e-3
-    for n in ("u", "v"):
-        assert np.array_equal(a.get_cg(n), b.get_cg(n)), n
-    for n in ("s11", "s12", "s22", "e11", "e12", "e22"):
-        assert np.array_equal(a.getDGData(n, S), b.getDGData(n, S)), n
+
+    def fields(k):
+        out = {n: k.get_cg(n) for n in ("u", "v")}
+        out.update({n: k.getDGData(n, S) for n in ("s11", "s12", "s22", "e11", "e12", "e22")})
+        return out
+
+    fa, fa2, fb = fields(a), fields(a2), fields(b)
+    for n in fa:
+        assert np.array_equal(fa[n], fa2[n]), n
+        err = np.abs(fa[n] - fb[n]).max() / max(np.abs(fb[n]).max(), 1e-300)
+        assert err <= 1e-14, (n, err)
 
 
 def test_pairing_mode_is_checked():
