@@ -200,9 +200,23 @@ int nsdg_dyn_benchmark_wind(nsdg_dyn* d, double L, double t_seconds, double xoff
 /* Zero the higher DG moments of a field on the device: the effect of the module-level
  * setData(name, getDG0Data(name)) round trip of MEVPDynamics::update (MEVPDynamics.cpp:64-65). */
 int nsdg_dyn_reset_dg0(nsdg_dyn* d, const char* name);
-/* FP64 conservation sums (Tools::MeanValue, Tools.hpp:24-45) of hice and cice over the
- * ocean elements, with compensated (two-sum) accumulation on the device. */
+/* FP64 conservation sums (Tools::MeanValue, Tools.hpp:24-45) of hice and cice over all
+ * elements (land included, as the reference sums them), with cos(latitude) weights on
+ * spherical meshes (Tools.hpp:28-36; totals in radians squared there), accumulated with
+ * compensated (two-sum) sums on the device in a fixed order. */
 int nsdg_dyn_mass(nsdg_dyn* d, double* ice_volume, double* ice_area);
+/* Advection mass budget (no counterpart in the reference, which does not conserve mass
+ * exactly: Dirichlet edges are outflow-only, DGTransport.cpp:306-351, and the limiters
+ * clamp the cell mean, dgLimit.hpp:16-84). With the budget on, every update/advect adds,
+ * per element, the Dirichlet outflow of h and of A over the step's RK stages and the
+ * change the limiters make to the Tools::MeanValue contribution of h and A.
+ * nsdg_dyn_budget writes the totals since the budget was switched on or last reset:
+ * out[0] outflow of h, out[1] outflow of A, out[2] limiter change of h, out[3] limiter
+ * change of A, in the units of nsdg_dyn_mass, so that over any run of steps
+ *   mass(after) - mass(before) = out[2] - out[0]  (h), out[3] - out[1]  (A)
+ * up to rounding. reset != 0 zeroes the totals after reading them. */
+int nsdg_dyn_set_budget(nsdg_dyn* d, int on);
+int nsdg_dyn_budget(nsdg_dyn* d, double* out, int reset);
 /* Momentum residual of the current velocity (no counterpart in the reference, which
  * runs a fixed subcycle count): R = M_L [c1 (u - u0) - A (tau_a + F_oc |u_o - u| (u_o - u))
  * + rho H f_c (u | -v) + rho H g grad(eta)] - dS(Pi sigma_VP(eps(u))) per CG node, the

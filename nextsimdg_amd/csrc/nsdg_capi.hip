@@ -423,6 +423,14 @@ int nsdg_dyn_mass(nsdg_dyn* d, double* vol, double* area)
 {
     return guard([&] { nonNull(d, "dyn")->k->mass(vol, area); });
 }
+int nsdg_dyn_set_budget(nsdg_dyn* d, int on)
+{
+    return guard([&] { nonNull(d, "dyn")->k->setBudget(on != 0); });
+}
+int nsdg_dyn_budget(nsdg_dyn* d, double* out, int reset)
+{
+    return guard([&] { nonNull(d, "dyn")->k->budget(nonNull(out, "out"), reset != 0); });
+}
 int nsdg_dyn_residual(nsdg_dyn* d, double* r, double* norm_r, double* norm_forcing)
 {
     return guard([&] { nonNull(d, "dyn")->k->residual(r, norm_r, norm_forcing); });

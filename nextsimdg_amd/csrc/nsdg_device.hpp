@@ -146,6 +146,89 @@ template <int D> __device__ void metricMassMatrix(const MeshView& m, const doubl
             M[i][j] = s;
         }
 }
+// Contribution of element e with DG coefficients c to Tools::MeanValue (Tools.hpp:24-45):
+// the field at the 3 x 3 Gauss points (whatever DG is) times J w, and times cos(latitude)
+// on the sphere (Tools.hpp:28-36), so spherical totals are in radians squared.
+template <int DG> __device__ double elementMean(const MeshView& m, int e, const double* c)
+{
+    double cc[4][2], J[9], cl[9];
+    elementCoords(m, e, cc);
+    jacobian<3>(cc, J);
+    metricWeight<3>(m, cc, cl);
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        double v = 0;
+#pragma unroll
+        for (int i = 0; i < DG; ++i)
+            v += c[i] * Psi<DG, 3>::v[i][q];
+        s += m.spherical ? v * cl[q] * (J[q] * W<3>::v[0][q]) : v * (J[q] * W<3>::v[0][q]);
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------- compensated sums
+// Two-sum: s + err == a + b exactly.
+__device__ __forceinline__ void twoSum(double a, double b, double& s, double& err)
+{
+    s = a + b;
+    const double bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+}
+// Compensated sum of NF values over a block of kBlock threads, in a fixed tree order
+// (deterministic). sum[f], comp[f] are each thread's running sum and its compensation;
+// thread 0 returns the block's pairs in sum, comp.
+template <int NF> __device__ void blockTwoSum(double (*sh)[kBlock], double* sum, double* comp)
+{
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        sh[2 * f][threadIdx.x] = sum[f];
+        sh[2 * f + 1][threadIdx.x] = comp[f];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                double s, err;
+                twoSum(sh[2 * f][threadIdx.x], sh[2 * f][threadIdx.x + w], s, err);
+                sh[2 * f][threadIdx.x] = s;
+                sh[2 * f + 1][threadIdx.x] += sh[2 * f + 1][threadIdx.x + w] + err;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        sum[f] = sh[2 * f][0];
+        comp[f] = sh[2 * f + 1][0];
+    }
+}
+// Final pass over nb block partials laid out part[(b * NF + f) * 2 + {0: sum, 1: comp}]:
+// one block; thread t takes the blocks t, t + kBlock, ... in order, then the block tree.
+// out[f] = sum + comp. Launch <<<1, kBlock>>>.
+template <int NF> __global__ __launch_bounds__(kBlock) void k_sum_final(const double* part, int nb, double* out)
+{
+    __shared__ double sh[2 * NF][kBlock];
+    double sum[NF], comp[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+        sum[f] = comp[f] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            double s, err;
+            twoSum(sum[f], part[(b * NF + f) * 2], s, err);
+            sum[f] = s;
+            comp[f] += err + part[(b * NF + f) * 2 + 1];
+        }
+    blockTwoSum<NF>(sh, sum, comp);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            out[f] = sum[f] + comp[f];
+}
+
 // In-register Gauss-Jordan inverse with partial pivoting (set-up kernels only).
 template <int N> __device__ void invert(double A[N][N], double R[N][N])
 {

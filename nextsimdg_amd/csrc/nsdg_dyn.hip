@@ -2008,8 +2008,10 @@ __global__ void k_boundaries(NodeArgs a)
 // c1 = rho H / dt, tau_a = F_atm |u_a| u_a. R = 0 exactly where the update
 // (VPCGDynamicsKernel.hpp:132-172) leaves u unchanged. Sums of R^2 and of (M_L A tau_a)^2
 // over the non-Dirichlet nodes: per-thread FP64 sums, wavefront shuffles, then one
-// partial per block, added in block order by k_residual_final (deterministic).
+// partial per block; k_sum_final<2> adds the block partials in a fixed tree order
+// (deterministic).
 constexpr int kResidualBlocks = 512;
+constexpr unsigned kTallyBlocks = 1024;
 __device__ __forceinline__ double waveSum(double x)
 {
 #pragma unroll
@@ -2058,21 +2060,12 @@ __global__ __launch_bounds__(kBlock) void k_residual_nodes(NodeArgs a, const dou
             sr += red[0][k];
             sd += red[1][k];
         }
-        part[2 * blockIdx.x] = sr;
-        part[2 * blockIdx.x + 1] = sd;
+        // the (sum, compensation) layout of k_sum_final<2>, compensation zero
+        part[4 * blockIdx.x] = sr;
+        part[4 * blockIdx.x + 1] = 0.0;
+        part[4 * blockIdx.x + 2] = sd;
+        part[4 * blockIdx.x + 3] = 0.0;
     }
-}
-__global__ void k_residual_final(const double* part, int nb, double* out)
-{
-    if (blockIdx.x != 0 || threadIdx.x != 0)
-        return;
-    double sr = 0.0, sd = 0.0;
-    for (int b = 0; b < nb; ++b) {
-        sr += part[2 * b];
-        sd += part[2 * b + 1];
-    }
-    out[0] = sr;
-    out[1] = sd;
 }
 
 // ---------------------------------------------------------------- learned warm start
@@ -2205,68 +2198,57 @@ __global__ void k_zero_higher(double* f, int ne, int nc)
     f[ne + t] = 0.0;
 }
 
-// Compensated (two-sum) block reduction of the element integrals of h and A
-// (Tools::MeanValue, Tools.hpp:24-45: 3x3 Gauss, J w weights, every element).
-__device__ __forceinline__ void twoSum(double a, double b, double& s, double& err)
-{
-    s = a + b;
-    const double bb = s - a;
-    err = (a - (s - bb)) + (b - bb);
-}
+// Element integrals of h and A (Tools::MeanValue, Tools.hpp:24-45: 3x3 Gauss, J w weights
+// and cos(latitude) on the sphere, every element), summed with compensation: per block a
+// (sum, compensation) pair per field, then k_sum_final<2> over the blocks in a fixed order.
 template <int DG>
 __global__ __launch_bounds__(kBlock) void k_mass_partial(MeshView m, int ne, const double* h, const double* A, double* part)
 {
     __shared__ double sh[4][kBlock];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    double vh = 0, va = 0;
+    double sum[2] = { 0.0, 0.0 }, comp[2] = { 0.0, 0.0 };
     if (e < ne) {
-        double c[4][2], J[9];
-        elementCoords(m, e, c);
-        jacobian<3>(c, J);
-        for (int q = 0; q < 9; ++q) {
-            double ph = 0, pa = 0;
-            for (int i = 0; i < DG; ++i) {
-                ph += h[static_cast<size_t>(i) * ne + e] * Psi<DG, 3>::v[i][q];
-                pa += A[static_cast<size_t>(i) * ne + e] * Psi<DG, 3>::v[i][q];
-            }
-            vh += ph * (J[q] * W<3>::v[0][q]);
-            va += pa * (J[q] * W<3>::v[0][q]);
-        }
+        double c[DG];
+#pragma unroll
+        for (int i = 0; i < DG; ++i)
+            c[i] = h[static_cast<size_t>(i) * ne + e];
+        sum[0] = elementMean<DG>(m, e, c);
+#pragma unroll
+        for (int i = 0; i < DG; ++i)
+            c[i] = A[static_cast<size_t>(i) * ne + e];
+        sum[1] = elementMean<DG>(m, e, c);
     }
-    sh[0][threadIdx.x] = vh;
-    sh[1][threadIdx.x] = 0.0;
-    sh[2][threadIdx.x] = va;
-    sh[3][threadIdx.x] = 0.0;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            for (int f = 0; f < 2; ++f) {
-                double s, err;
-                twoSum(sh[2 * f][threadIdx.x], sh[2 * f][threadIdx.x + w], s, err);
-                sh[2 * f][threadIdx.x] = s;
-                sh[2 * f + 1][threadIdx.x] += sh[2 * f + 1][threadIdx.x + w] + err;
-            }
-        }
-        __syncthreads();
-    }
+    blockTwoSum<2>(sh, sum, comp);
     if (threadIdx.x == 0)
-        for (int f = 0; f < 4; ++f)
-            part[4 * blockIdx.x + f] = sh[f][0];
-}
-__global__ void k_mass_final(const double* part, int nb, double* out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
-    for (int f = 0; f < 2; ++f) {
-        double s = 0, c = 0;
-        for (int b = 0; b < nb; ++b) {
-            double t, err;
-            twoSum(s, part[4 * b + 2 * f], t, err);
-            s = t;
-            c += err + part[4 * b + 2 * f + 1];
+        for (int f = 0; f < 2; ++f) {
+            part[(blockIdx.x * 2 + f) * 2] = sum[f];
+            part[(blockIdx.x * 2 + f) * 2 + 1] = comp[f];
         }
-        out[f] = s + c;
-    }
+}
+// Compensated sums of NF per-element arrays x[f * n + e] (the advection budget tallies),
+// block partials for k_sum_final<NF>; each thread takes the elements t, t + grid, ...
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_tally_partial(const double* x, int n, double* part)
+{
+    __shared__ double sh[2 * NF][kBlock];
+    double sum[NF], comp[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+        sum[f] = comp[f] = 0.0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            double s, err;
+            twoSum(sum[f], x[static_cast<size_t>(f) * n + e], s, err);
+            sum[f] = s;
+            comp[f] += err;
+        }
+    blockTwoSum<NF>(sh, sum, comp);
+    if (threadIdx.x == 0)
+        for (int f = 0; f < NF; ++f) {
+            part[(blockIdx.x * NF + f) * 2] = sum[f];
+            part[(blockIdx.x * NF + f) * 2 + 1] = comp[f];
+        }
 }
 
 // ---------------------------------------------------------------- host object
@@ -2979,6 +2961,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     DBuf<float> sigFAlt;            // ping-pong partner of sigF (fused subcycle)
     uint32_t subcycleCount = 0;     // element passes run so far (FP32 stochastic rounding)
     DBuf<double> sigVP, rpart, rsum; // momentum residual: VP stress, block partials, sums
+    bool budgetOn = false;           // advection mass budget: per-element tallies [4][ne]
+    DBuf<double> tally;
     DBuf<double> uStart, vStart, sigStart; // saved start of a timestep (warm-start gate)
     DBuf<double> uAlt, vAlt, sigAlt;       // ping-pong partners of u, v, sigma (fused subcycle)
     std::map<std::string, std::pair<int, std::vector<double>>> passive;
@@ -3018,6 +3002,11 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         nn1 = static_cast<size_t>(mesh.nx + 1) * (mesh.ny + 1);
         // the kernels index CG nodes with 32-bit ints (the mesh checks elements and vertices)
         require(nn < (size_t(1) << 31), "too many CG nodes for 32-bit node indices", NSDG_ERR_ARG);
+        if (!mesh.uniform) {
+            // the one-pass kernel addresses element fields and CG nodes with 32-bit byte offsets
+            require(ne < (size_t(1) << 29), "general meshes need fewer than 2^29 elements", NSDG_ERR_ARG);
+            require(nn < (size_t(1) << 29), "general meshes need fewer than 2^29 CG nodes", NSDG_ERR_ARG);
+        }
         tr = makeTransport(DG, mesh, s, prof);
         tr->setScheme(2); // DynamicsKernel.hpp:61
         cg2dg.build(CG, DG, mesh, s);
@@ -3026,8 +3015,6 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         k_momentum_ops<CG><<<blocksFor(nOps), kBlock, 0, s>>>(view(mesh), nOps, mops.p);
         checkLaunch("momentum_ops");
         if (!mesh.uniform) {
-            // the one-pass kernel addresses element fields with 32-bit byte offsets
-            require(ne < (size_t(1) << 29), "general meshes need fewer than 2^29 elements", NSDG_ERR_ARG);
             geo.alloc(static_cast<size_t>(GeoLayout<CG>::K) * ne);
             k_geometry<CG><<<blocksFor(ne), kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), geo.p);
             checkLaunch("geometry");
@@ -3052,14 +3039,18 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         for (auto* b : { &fzOcn, &fzCf, &fzQ })
             b->alloc(nn);
         strength.alloc(static_cast<size_t>(L::Q) * ne);
-        part.alloc(4 * blocksFor(ne));
-        res.alloc(2);
+        part.alloc(std::max<size_t>(4 * blocksFor(ne), 8 * kTallyBlocks));
+        res.alloc(4);
+        if (budgetOn) { // a re-initialised mesh starts a new budget
+            tally.alloc(4 * ne);
+            deviceFill(tally.p, tally.n, 0.0, s);
+        }
         sigVP.alloc(3 * S * ne);
         deviceFill(sigVP.p, sigVP.n, 0.0, s);
         uAlt.alloc(nn);
         vAlt.alloc(nn);
         sigAlt.alloc(3 * S * ne);
-        rpart.alloc(2 * kResidualBlocks);
+        rpart.alloc(4 * kResidualBlocks);
         rsum.alloc(2);
         for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
             deviceFill(b->p, b->n, 0.0, s);
@@ -3301,14 +3292,17 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // DynamicsKernel::advectionAndLimits (DynamicsKernel.hpp:160-172); the brittle kernel
     // advects with the velocity averaged over the previous step's substeps
     // (BrittleCGDynamicsKernel.hpp:89-90).
+    // With the mass budget on, the Dirichlet outflow and the limiters' change of h and A
+    // are tallied per element (tally: outflow h, outflow A, limiter h, limiter A).
     void advectAsync(double dt)
     {
         cg2dg.apply(mesh, brittle ? avgU.p : u.p, tr->velx(), brittle ? avgV.p : v.p, tr->vely(), s, prof);
         tr->reinitNormalVelocity();
-        tr->step(dt, A.p);
-        tr->step(dt, h.p);
-        tr->limit(A.p, true, 1.0, true, 0.0);
-        tr->limit(h.p, false, 0.0, true, 0.0);
+        double* t = budgetOn ? tally.p : nullptr;
+        tr->step(dt, A.p, t ? t + ne : nullptr);
+        tr->step(dt, h.p, t);
+        tr->limit(A.p, true, 1.0, true, 0.0, t ? t + 3 * ne : nullptr);
+        tr->limit(h.p, false, 0.0, true, 0.0, t ? t + 2 * ne : nullptr);
     }
     void prepareIterationAsync()
     {
@@ -3905,7 +3899,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             }
             k_residual_nodes<<<kResidualBlocks, kBlock, 0, s>>>(a, tmpCG.p, tmpCG2.p, rpart.p);
             checkLaunch("residual_nodes");
-            k_residual_final<<<1, 1, 0, s>>>(rpart.p, kResidualBlocks, rsum.p);
+            k_sum_final<2><<<1, kBlock, 0, s>>>(rpart.p, kResidualBlocks, rsum.p);
             checkLaunch("residual_final");
         }
         double h2[2];
@@ -4060,7 +4054,7 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         const unsigned nb = blocksFor(ne);
         k_mass_partial<DG><<<nb, kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), h.p, A.p, part.p);
         checkLaunch("mass_partial");
-        k_mass_final<<<1, 1, 0, s>>>(part.p, static_cast<int>(nb), res.p);
+        k_sum_final<2><<<1, kBlock, 0, s>>>(part.p, static_cast<int>(nb), res.p);
         checkLaunch("mass_final");
         double r[2];
         res.download(r, 2, s);
@@ -4069,6 +4063,36 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             *vol = r[0];
         if (area)
             *area = r[1];
+    }
+    // ---- advection mass budget (no counterpart in the reference)
+    void setBudget(bool on) override
+    {
+        need();
+        if (on && !budgetOn) {
+            tally.alloc(4 * ne);
+            deviceFill(tally.p, tally.n, 0.0, s);
+        }
+        if (!on)
+            tally.release();
+        budgetOn = on;
+        NSDG_HIP(hipStreamSynchronize(s));
+    }
+    void budget(double* out, bool reset) override
+    {
+        need();
+        require(budgetOn, "the mass budget is off (nsdg_dyn_set_budget)", NSDG_ERR_STATE);
+        const unsigned nb = std::min<unsigned>(blocksFor(ne), kTallyBlocks);
+        k_tally_partial<4><<<nb, kBlock, 0, s>>>(tally.p, static_cast<int>(ne), part.p);
+        checkLaunch("tally_partial");
+        k_sum_final<4><<<1, kBlock, 0, s>>>(part.p, static_cast<int>(nb), res.p);
+        checkLaunch("tally_final");
+        double r[4];
+        res.download(r, 4, s);
+        if (reset)
+            deviceFill(tally.p, tally.n, 0.0, s);
+        NSDG_HIP(hipStreamSynchronize(s));
+        if (out)
+            std::copy(r, r + 4, out);
     }
     void synchronize() override
     {

@@ -36,6 +36,8 @@ struct TrArgs {
     double* out;
     double* raw;        // un-finished increment (periodic path)
     bool withDirichlet;
+    double* tally;      // per-element Dirichlet outflow accumulator (mass budget), or null
+    double tallyW;      // weight of this stage's outflow in the step (1/R folded in on the sphere)
 };
 
 // ---------------------------------------------------------------- operator set-up
@@ -255,8 +257,10 @@ __device__ __forceinline__ void upwind(const double* p1, const double* p2, const
         tmp[q] = fmax(vg[q], 0.0) * g1[q] + fmin(vg[q], 0.0) * g2[q];
 }
 // Dirichlet outflow-only boundary term (DGTransport.cpp:306-351), applied count times.
+// Returns the outflow: count times the flux tested with the constant basis function, the
+// amount coefficient 0 loses before the inverse mass.
 template <int DG, int SIDE>
-__device__ __forceinline__ void boundaryTerm(double* up, const double* ph, const double* nv, double dt, int count)
+__device__ __forceinline__ double boundaryTerm(double* up, const double* ph, const double* nv, double dt, int count)
 {
     constexpr int E = edgedofs(DG), G = gp1d(DG);
     double tr[E], pg[G], vg[G], tmp[G];
@@ -269,7 +273,19 @@ __device__ __forceinline__ void boundaryTerm(double* up, const double* ph, const
         tmp[q] = pg[q] * fmax(sgn * vg[q], 0.0);
     for (int r = 0; r < count; ++r)
         edgeAccumulate<DG, SIDE>(up, -1.0, dt, tmp);
+    double o = 0;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        const double w = SIDE == 0 ? PsiEW<DG, G>::b[q][0]
+            : SIDE == 1            ? PsiEW<DG, G>::r[q][0]
+            : SIDE == 2            ? PsiEW<DG, G>::t[q][0]
+                                   : PsiEW<DG, G>::l[q][0];
+        o += dt * tmp[q] * w;
+    }
+    return count * o;
 }
+// All Dirichlet terms of element e; with a.tally set, the stage-weighted outflow is added
+// to the element's budget accumulator (elements without Dirichlet sides add nothing).
 template <int DG> __device__ __forceinline__ void dirichletTerms(const TrArgs& a, int e, const double* ph, double* up)
 {
     constexpr int E = edgedofs(DG);
@@ -277,23 +293,25 @@ template <int DG> __device__ __forceinline__ void dirichletTerms(const TrArgs& a
     if (!fl)
         return;
     const int nx = a.m.nx, ix = e % nx, iy = e / nx;
-    double nv[E];
+    double nv[E], out = 0;
     if (fl & 0xffu) {
         loadEdge<E>(a.nvX, a.nXe, nx * iy + ix, nv);
-        boundaryTerm<DG, 0>(up, ph, nv, a.dt, fl & 0xffu);
+        out += boundaryTerm<DG, 0>(up, ph, nv, a.dt, fl & 0xffu);
     }
     if (fl & 0xff00u) {
         loadEdge<E>(a.nvY, a.nYe, (nx + 1) * iy + ix + 1, nv);
-        boundaryTerm<DG, 1>(up, ph, nv, a.dt, (fl >> 8) & 0xffu);
+        out += boundaryTerm<DG, 1>(up, ph, nv, a.dt, (fl >> 8) & 0xffu);
     }
     if (fl & 0xff0000u) {
         loadEdge<E>(a.nvX, a.nXe, nx * (iy + 1) + ix, nv);
-        boundaryTerm<DG, 2>(up, ph, nv, a.dt, (fl >> 16) & 0xffu);
+        out += boundaryTerm<DG, 2>(up, ph, nv, a.dt, (fl >> 16) & 0xffu);
     }
     if (fl & 0xff000000u) {
         loadEdge<E>(a.nvY, a.nYe, (nx + 1) * iy + ix, nv);
-        boundaryTerm<DG, 3>(up, ph, nv, a.dt, fl >> 24);
+        out += boundaryTerm<DG, 3>(up, ph, nv, a.dt, fl >> 24);
     }
+    if (a.tally)
+        a.tally[e] += a.tallyW * out;
 }
 // Inverse mass matrix and Runge-Kutta stage combination (DGTransport.cpp:509-552).
 template <int DG, bool UNI> __device__ __forceinline__ void finishCell(const TrArgs& a, int e, const double* ph, const double* up)
@@ -473,15 +491,20 @@ template <int DG, bool UNI> __global__ __launch_bounds__(kBlock) void k_transpor
 }
 
 // ---------------------------------------------------------------- limiters
-// LimitMax then LimitMin (dgLimit.hpp:16-84) on one SoA field.
+// LimitMax then LimitMin (dgLimit.hpp:16-84) on one SoA field. With tally set, the change
+// of the element's Tools::MeanValue contribution is added to tally[e] (mass budget).
 template <int DG>
-__global__ void k_limit(double* f, int ne, bool doMax, double maxv, bool doMin, double minv)
+__global__ void k_limit(MeshView m, double* f, int ne, bool doMax, double maxv, bool doMin, double minv,
+    double* tally)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne)
         return;
-    double c[DG];
+    double c[DG], c0[DG];
     loadCell<DG>(f, ne, e, c);
+#pragma unroll
+    for (int i = 0; i < DG; ++i)
+        c0[i] = c[i];
     for (int pass = 0; pass < 2; ++pass) {
         const bool isMax = pass == 0;
         if (isMax ? !doMax : !doMin)
@@ -520,6 +543,14 @@ __global__ void k_limit(double* f, int ne, bool doMax, double maxv, bool doMin, 
 #pragma unroll
     for (int i = 0; i < DG; ++i)
         f[static_cast<size_t>(i) * ne + e] = c[i];
+    if (tally) {
+        bool changed = false;
+#pragma unroll
+        for (int i = 0; i < DG; ++i)
+            changed = changed || c[i] != c0[i];
+        if (changed)
+            tally[e] += elementMean<DG>(m, e, c) - elementMean<DG>(m, e, c0);
+    }
 }
 
 // ---------------------------------------------------------------- CG -> DG
@@ -821,48 +852,58 @@ template <int DG> struct TransportImpl final : TransportBase {
             k_transport_finish<DG, false><<<nb, kBlock, 0, s>>>(a);
         checkLaunch("transport_finish");
     }
-    // step_rk1 / step_rk2 / step_rk3 (DGTransport.cpp:514-552) on a device field.
-    void step(double dt, double* phi) override
+    // step_rk1 / step_rk2 / step_rk3 (DGTransport.cpp:514-552) on a device field. With
+    // tally set, each element's Dirichlet outflow over the step (in MeanValue units) is
+    // added to tally[e]: the stage outflows weighted as the stages enter the new field.
+    void step(double dt, double* phi, double* tally) override
     {
         const size_t n = mesh.ne * DG;
+        // on the sphere the inverse mass carries 1/R (ParametricMap.cpp:68-87)
+        const double unit = mesh.spherical ? 1.0 / kEarthRadius : 1.0;
+        auto withTally = [&](TrArgs a, double w) {
+            a.tally = tally;
+            a.tallyW = w * unit;
+            return a;
+        };
         if (scheme == 1) {
-            TrArgs a = args(dt, phi);
+            TrArgs a = withTally(args(dt, phi), 1.0);
             a.mode = ST_PLUS;
             a.out = tmpA.p;
             stage(a);
             deviceCopy(phi, tmpA.p, n, s);
-        } else if (scheme == 2) {
-            TrArgs a = args(dt, phi);
+        } else if (scheme == 2) { // phi + (k1 + k2) / 2
+            TrArgs a = withTally(args(dt, phi), 0.5);
             a.mode = ST_PLUS_K;
             a.kout = kbuf.p;
             a.out = tmpA.p;
             stage(a);
-            TrArgs b = args(dt, tmpA.p);
+            TrArgs b = withTally(args(dt, tmpA.p), 0.5);
             b.mode = ST_RK2B;
             b.kin = kbuf.p;
             b.out = phi;
             stage(b);
-        } else {
-            TrArgs a = args(dt, phi);
+        } else { // phi + (k1 + k2) / 6 + 2 k3 / 3
+            TrArgs a = withTally(args(dt, phi), 1.0 / 6.0);
             a.mode = ST_PLUS;
             a.out = tmpA.p;
             stage(a);
-            TrArgs b = args(dt, tmpA.p);
+            TrArgs b = withTally(args(dt, tmpA.p), 1.0 / 6.0);
             b.mode = ST_RK3B;
             b.base = phi;
             b.out = tmpB.p;
             stage(b);
-            TrArgs c = args(dt, tmpB.p);
+            TrArgs c = withTally(args(dt, tmpB.p), 2.0 / 3.0);
             c.mode = ST_RK3C;
             c.base = phi;
             c.out = phi;
             stage(c);
         }
     }
-    void limit(double* phi, bool doMax, double maxv, bool doMin, double minv) override
+    void limit(double* phi, bool doMax, double maxv, bool doMin, double minv, double* tally) override
     {
         Timed t(prof, s, "limit");
-        k_limit<DG><<<blocksFor(mesh.ne), kBlock, 0, s>>>(phi, static_cast<int>(mesh.ne), doMax, maxv, doMin, minv);
+        k_limit<DG><<<blocksFor(mesh.ne), kBlock, 0, s>>>(
+            view(mesh), phi, static_cast<int>(mesh.ne), doMax, maxv, doMin, minv, tally);
         checkLaunch("limit");
     }
 };

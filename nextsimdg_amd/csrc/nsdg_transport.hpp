@@ -14,8 +14,11 @@ struct TransportBase {
     virtual void setVelocityHost(const double* vx, const double* vy) = 0; // N_e x DG host layout
     virtual void prepareAdvectionHost(int cg, const double* u, const double* v) = 0;
     virtual void reinitNormalVelocity() = 0;
-    virtual void step(double dt, double* phiDev) = 0;                      // SoA field on the device
-    virtual void limit(double* phiDev, bool doMax, double maxv, bool doMin, double minv) = 0;
+    // SoA field on the device; tally (optional, N_e doubles) accumulates each element's
+    // Dirichlet outflow in Tools::MeanValue units
+    virtual void step(double dt, double* phiDev, double* tally = nullptr) = 0;
+    // tally (optional) accumulates the change of each element's MeanValue contribution
+    virtual void limit(double* phiDev, bool doMax, double maxv, bool doMin, double minv, double* tally = nullptr) = 0;
     virtual double* velx() = 0;
     virtual double* vely() = 0;
 };

@@ -268,9 +268,22 @@ class MEVPDynamicsKernel:
         check(self.lib.nsdg_dyn_reset_dg0(self.h, name.encode()))
 
     def mass(self):
+        """Tools::MeanValue of hice and cice (Tools.hpp:24-45): (ice volume, ice area)."""
         v, a = ctypes.c_double(), ctypes.c_double()
         check(self.lib.nsdg_dyn_mass(self.h, ctypes.byref(v), ctypes.byref(a)))
         return v.value, a.value
+
+    def set_budget(self, on=True):
+        """Switch the advection mass budget on (zeroed) or off."""
+        check(self.lib.nsdg_dyn_set_budget(self.h, int(bool(on))))
+
+    def budget(self, reset=False):
+        """Advection mass budget since it was switched on or reset: the Dirichlet outflow
+        and the limiters' change of the MeanValue of h and A, such that
+        mass(after) - mass(before) = limiter - outflow for each."""
+        out = np.zeros(4)
+        check(self.lib.nsdg_dyn_budget(self.h, dptr(out), int(bool(reset))))
+        return dict(outflow_h=out[0], outflow_A=out[1], limiter_h=out[2], limiter_A=out[3])
 
     def momentum_residual(self):
         """(r, |R|, |M_L A tau_a|): the fixed-point residual of the momentum update with the

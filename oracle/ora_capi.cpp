@@ -189,6 +189,8 @@ struct DynBase {
     virtual size_t ncg() = 0;
     virtual void setRheology(int) = 0;
     virtual MEBParams& meb() = 0;
+    virtual void mass(double* out) = 0;
+    virtual double* budget() = 0;
 };
 template <int CG, int DG> struct DynT : DynBase {
     Kernel<CG, DG> k;
@@ -240,6 +242,12 @@ template <int CG, int DG> struct DynT : DynBase {
         }
     }
     size_t ncg() override { return k.ncg(); }
+    void mass(double* out) override
+    {
+        out[0] = meanValue<DG>(k.mesh, k.hice.data());
+        out[1] = meanValue<DG>(k.mesh, k.cice.data());
+    }
+    double* budget() override { return k.budget; }
 };
 
 DynBase* makeDyn(int cg, int dg)
@@ -280,9 +288,9 @@ const char* ora_last_error() { return g_err.c_str(); }
 // ------------------------------------------------------------------ mesh
 void* ora_mesh_new() { return new Mesh(); }
 void ora_mesh_free(void* m) { delete static_cast<Mesh*>(m); }
-int ora_mesh_from_coords(void* m, size_t nx, size_t ny, const double* coords, const double* mask)
+int ora_mesh_from_coords(void* m, size_t nx, size_t ny, const double* coords, const double* mask, int spherical)
 {
-    return guard([&] { static_cast<Mesh*>(m)->fromCoordsAndMask(nx, ny, coords, mask); });
+    return guard([&] { static_cast<Mesh*>(m)->fromCoordsAndMask(nx, ny, coords, mask, spherical != 0); });
 }
 int ora_mesh_rectangle(void* m, double Lx, double Ly, size_t nx, size_t ny, double distort)
 {
@@ -588,6 +596,19 @@ int ora_dyn_update(void* d, double dt) { return guard([&] { static_cast<DynBase*
 int ora_dyn_advection(void* d, double dt) { return guard([&] { static_cast<DynBase*>(d)->advection(dt); }); }
 int ora_dyn_prepare_iteration(void* d) { return guard([&] { static_cast<DynBase*>(d)->prepareIteration(); }); }
 int ora_dyn_begin_iteration(void* d, double dt) { return guard([&] { static_cast<DynBase*>(d)->beginIteration(dt); }); }
+// Tools::MeanValue of hice and cice (ice volume and area)
+int ora_dyn_mass(void* d, double* out) { return guard([&] { static_cast<DynBase*>(d)->mass(out); }); }
+// Advection mass budget since the last reset: outflow h, outflow A, limiter h, limiter A
+int ora_dyn_budget(void* d, double* out, int reset)
+{
+    return guard([&] {
+        double* b = static_cast<DynBase*>(d)->budget();
+        if (out)
+            std::copy(b, b + 4, out);
+        if (reset)
+            std::fill(b, b + 4, 0.0);
+    });
+}
 // which: 0 strain, 1 stress, 2 divergence, 3 momentum, 4 boundaries, 5 whole subcycle
 int ora_dyn_pass(void* d, int which, int repeat)
 {

@@ -475,8 +475,10 @@ template <int DG> struct Transport {
         edgeAdd(&phiup[c1 * DG], -1.0, dt, tmp, isX ? TOP : RIGHT);
         edgeAdd(&phiup[c2 * DG], +1.0, dt, tmp, isX ? BOTTOM : LEFT);
     }
-    // boundary_lower / boundary_upper / boundary_left / boundary_right (DGTransport.cpp:306-351)
-    void boundary(int side, double dt, double* phiup, const double* phi, size_t c, size_t e) const
+    // boundary_lower / boundary_upper / boundary_left / boundary_right (DGTransport.cpp:306-351).
+    // Returns the outflow through the edge: the flux tested with the constant basis function,
+    // i.e. what the cell's coefficient 0 loses before the inverse mass.
+    double boundary(int side, double dt, double* phiup, const double* phi, size_t c, size_t e) const
     {
         const bool isX = (side == BOTTOM || side == TOP);
         double vg[G], tr[E], pg[G], tmp[G];
@@ -487,9 +489,18 @@ template <int DG> struct Transport {
         for (int q = 0; q < G; ++q)
             tmp[q] = pg[q] * std::max(sgn * vg[q], 0.0);
         edgeAdd(&phiup[c * DG], -1.0, dt, tmp, side);
+        const auto& PW = PsiEWT<DG, G>::get();
+        double o = 0;
+        for (int q = 0; q < G; ++q)
+            o += dt * tmp[q] * PW.v[side][q][0];
+        return o;
     }
-    // DGTransportOperator (DGTransport.cpp:435-512)
-    void op(double dt, const double* phi, double* phiup) const
+    // Outflow through the Dirichlet edges over the steps taken since it was last zeroed, in
+    // the units of Tools::MeanValue: sum over RK stages of the stage weight times the
+    // stage's outflow (times 1/R on the sphere, where the inverse mass carries it).
+    double outflow = 0;
+    // DGTransportOperator (DGTransport.cpp:435-512); *out0 receives the Dirichlet outflow
+    void op(double dt, const double* phi, double* phiup, double* out0 = nullptr) const
     {
         const size_t nx = m.nx, ny = m.ny, ne = m.nelements;
         std::fill(phiup, phiup + ne * DG, 0.0);
@@ -518,21 +529,24 @@ template <int DG> struct Transport {
                     edge_term(false, dt, phiup, phi, seg[i][1], seg[i][2], seg[i][3]);
             }
         }
+        double out = 0;
         for (int seg = 0; seg < 4; ++seg) {
             const auto& dl = m.dirichlet[seg];
-#pragma omp parallel for
+#pragma omp parallel for reduction(+ : out)
             for (size_t i = 0; i < dl.size(); ++i) {
                 const size_t eid = dl[i], ix = eid % nx, iy = eid / nx;
                 if (seg == 0)
-                    boundary(BOTTOM, dt, phiup, phi, eid, nx * iy + ix);
+                    out += boundary(BOTTOM, dt, phiup, phi, eid, nx * iy + ix);
                 else if (seg == 1)
-                    boundary(RIGHT, dt, phiup, phi, eid, (nx + 1) * iy + ix + 1);
+                    out += boundary(RIGHT, dt, phiup, phi, eid, (nx + 1) * iy + ix + 1);
                 else if (seg == 2)
-                    boundary(TOP, dt, phiup, phi, eid, nx * (iy + 1) + ix);
+                    out += boundary(TOP, dt, phiup, phi, eid, nx * (iy + 1) + ix);
                 else
-                    boundary(LEFT, dt, phiup, phi, eid, (nx + 1) * iy + ix);
+                    out += boundary(LEFT, dt, phiup, phi, eid, (nx + 1) * iy + ix);
             }
         }
+        if (out0)
+            *out0 = m.spherical ? out / EarthRadius : out;
 #pragma omp parallel for
         for (size_t eid = 0; eid < ne; ++eid) {
             double r[DG];
@@ -550,33 +564,37 @@ template <int DG> struct Transport {
     void step(double dt, double* phi)
     {
         const size_t n = m.nelements * DG;
+        double o1 = 0, o2 = 0, o3 = 0;
         if (scheme == 1) {
-            op(dt, phi, tmp1.data());
+            op(dt, phi, tmp1.data(), &o1);
             for (size_t i = 0; i < n; ++i)
                 phi[i] += tmp1[i];
+            outflow += o1;
         } else if (scheme == 2) {
-            op(dt, phi, tmp1.data());
+            op(dt, phi, tmp1.data(), &o1);
             for (size_t i = 0; i < n; ++i)
                 phi[i] += tmp1[i];
-            op(dt, phi, tmp2.data());
+            op(dt, phi, tmp2.data(), &o2);
             for (size_t i = 0; i < n; ++i)
                 phi[i] += 0.5 * (tmp2[i] - tmp1[i]);
+            outflow += 0.5 * o1 + 0.5 * o2; // phi + (k1 + k2) / 2
         } else if (scheme == 3) {
-            op(dt, phi, tmp1.data());
+            op(dt, phi, tmp1.data(), &o1);
             for (size_t i = 0; i < n; ++i)
                 tmp1[i] += phi[i];
-            op(dt, tmp1.data(), tmp2.data());
+            op(dt, tmp1.data(), tmp2.data(), &o2);
             for (size_t i = 0; i < n; ++i) {
                 tmp2[i] += tmp1[i];
                 tmp2[i] *= 0.25;
                 tmp2[i] += 0.75 * phi[i];
             }
-            op(dt, tmp2.data(), tmp3.data());
+            op(dt, tmp2.data(), tmp3.data(), &o3);
             for (size_t i = 0; i < n; ++i) {
                 tmp3[i] += tmp2[i];
                 phi[i] *= 1.0 / 3.0;
                 phi[i] += 2.0 / 3.0 * tmp3[i];
             }
+            outflow += (o1 + o2) / 6.0 + 2.0 / 3.0 * o3; // phi + (k1 + k2) / 6 + 2 k3 / 3
         } else
             throw std::runtime_error("unknown time stepping scheme");
     }
@@ -846,24 +864,32 @@ template <int DG> inline double L2ErrorFunctionDG(const Mesh& m, const double* s
     return error;
 }
 
-// Tools::MeanValue (dynamics/src/include/Tools.hpp:24-45): 3x3 Gauss regardless of DG
-template <int DG> inline double meanValue(const Mesh& m, const double* phi)
+// Tools::MeanValue (dynamics/src/include/Tools.hpp:24-45): 3x3 Gauss regardless of DG;
+// on spherical meshes each Gauss point is weighted by cos(latitude) as well (:28-36), so
+// the result is the integral over the unit sphere (radians squared). meanValueCell is the
+// contribution of element i with coefficients c.
+template <int DG> inline double meanValueCell(const Mesh& m, size_t i, const double* c)
 {
     const auto& P = PsiT<DG, 3>::get();
     const auto& W = WT<3>::get();
-    double mass = 0;
-    for (size_t i = 0; i < m.nelements; ++i) {
-        double J[9];
-        jacobian<3>(m, i, J);
-        double s = 0;
-        for (int q = 0; q < 9; ++q) {
-            double v = 0;
-            for (int k = 0; k < DG; ++k)
-                v += phi[i * DG + k] * P.v[k][q];
-            s += v * (J[q] * W.w[q]);
-        }
-        mass += s;
+    double J[9], gx[9], gy[9];
+    jacobian<3>(m, i, J);
+    if (m.spherical)
+        gaussPoints<3>(m, i, gx, gy);
+    double s = 0;
+    for (int q = 0; q < 9; ++q) {
+        double v = 0;
+        for (int k = 0; k < DG; ++k)
+            v += c[k] * P.v[k][q];
+        s += m.spherical ? v * std::cos(gy[q]) * (J[q] * W.w[q]) : v * (J[q] * W.w[q]);
     }
+    return s;
+}
+template <int DG> inline double meanValue(const Mesh& m, const double* phi)
+{
+    double mass = 0;
+    for (size_t i = 0; i < m.nelements; ++i)
+        mass += meanValueCell<DG>(m, i, phi + i * DG);
     return mass;
 }
 
@@ -1365,11 +1391,31 @@ template <int CG, int DG> struct Kernel {
         CG2DG<CG, DG>(mesh, tr->velx.data(), ua.data());
         CG2DG<CG, DG>(mesh, tr->vely.data(), va.data());
         tr->reinitnormalvelocity();
+        tr->outflow = 0;
         tr->step(dt, cice.data());
+        budget[1] += tr->outflow;
+        tr->outflow = 0;
         tr->step(dt, hice.data());
+        budget[0] += tr->outflow;
+        const std::vector<double> a0 = cice, h0 = hice;
         limitMax<DG>(cice.data(), mesh.nelements, 1.0);
         limitMin<DG>(cice.data(), mesh.nelements, 0.0);
         limitMin<DG>(hice.data(), mesh.nelements, 0.0);
+        budget[2] += limiterChange(h0, hice);
+        budget[3] += limiterChange(a0, cice);
+    }
+    // Mass budget of the advection (not part of the reference): Dirichlet outflow of h
+    // and A, and the change the limiters make to their Tools::MeanValue, summed over the
+    // steps since it was zeroed. Over a step, MeanValue(after) - MeanValue(before) equals
+    // limiter change - outflow up to rounding: interior fluxes cancel pairwise.
+    double budget[4] = { 0, 0, 0, 0 }; // outflow h, outflow A, limiter h, limiter A
+    double limiterChange(const std::vector<double>& before, const std::vector<double>& after) const
+    {
+        double d = 0;
+        for (size_t i = 0; i < mesh.nelements; ++i)
+            if (!std::equal(&before[i * DG], &before[i * DG] + DG, &after[i * DG]))
+                d += meanValueCell<DG>(mesh, i, &after[i * DG]) - meanValueCell<DG>(mesh, i, &before[i * DG]);
+        return d;
     }
 
     // CGDynamicsKernel::ComputeGradientOfSeaSurfaceHeight (CGDynamicsKernel.cpp:125-258)

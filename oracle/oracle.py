@@ -41,7 +41,7 @@ def load():
         "ora_last_error": (ctypes.c_char_p, []),
         "ora_mesh_new": (_vp, []),
         "ora_mesh_free": (None, [_vp]),
-        "ora_mesh_from_coords": (ctypes.c_int, [_vp, _sz, _sz, _dp, _dp]),
+        "ora_mesh_from_coords": (ctypes.c_int, [_vp, _sz, _sz, _dp, _dp, ctypes.c_int]),
         "ora_mesh_rectangle": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _sz, _sz, ctypes.c_double]),
         "ora_mesh_ring": (ctypes.c_int, [_vp, _sz, _sz]),
         "ora_mesh_read_smesh": (ctypes.c_int, [_vp, ctypes.c_char_p]),
@@ -82,6 +82,8 @@ def load():
         "ora_dyn_prepare_iteration": (ctypes.c_int, [_vp]),
         "ora_dyn_begin_iteration": (ctypes.c_int, [_vp, ctypes.c_double]),
         "ora_dyn_pass": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+        "ora_dyn_mass": (ctypes.c_int, [_vp, _dp]),
+        "ora_dyn_budget": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
         "ora_benchmark_wind": (None, [_sz, _sz, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, _dp, _dp]),
         "ora_table": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
@@ -120,10 +122,10 @@ class OracleMesh:
             self.h = None
 
     @classmethod
-    def from_coords(cls, nx, ny, coords, mask):
+    def from_coords(cls, nx, ny, coords, mask, spherical=False):
         m = cls()
         c, k = _f64(coords), _f64(mask)
-        _check(m.lib.ora_mesh_from_coords(m.h, nx, ny, _p(c), _p(k)))
+        _check(m.lib.ora_mesh_from_coords(m.h, nx, ny, _p(c), _p(k), int(bool(spherical))))
         return m
 
     @classmethod
@@ -316,6 +318,19 @@ class OracleDyn:
 
     def run_pass(self, name, repeat=1):
         _check(self.lib.ora_dyn_pass(self.h, self.PASSES[name], repeat))
+
+    def mass(self):
+        """Tools::MeanValue of hice and cice: (ice volume, ice area)."""
+        out = np.zeros(2)
+        _check(self.lib.ora_dyn_mass(self.h, _p(out)))
+        return float(out[0]), float(out[1])
+
+    def budget(self, reset=False):
+        """Advection mass budget since the last reset, as a dict with the Dirichlet outflow
+        and the limiters' change of the MeanValue of h and A."""
+        out = np.zeros(4)
+        _check(self.lib.ora_dyn_budget(self.h, _p(out), int(bool(reset))))
+        return dict(outflow_h=out[0], outflow_A=out[1], limiter_h=out[2], limiter_A=out[3])
 
 
 class Oracle:

@@ -1,0 +1,148 @@
+"""Conservation on the GPU: Tools::MeanValue sums (with the spherical cos(latitude) branch,
+Tools.hpp:24-45) and the advection mass budget.
+
+  * nsdg_dyn_mass equals the oracle's MeanValue on a spherical patch (1e-13) and a
+    constant field gives the rectangle's area on the unit sphere;
+  * the budget tallies (Dirichlet outflow, DGTransport.cpp:306-351; limiter change,
+    dgLimit.hpp:16-84) equal the oracle's after module-level steps of the benchmark box
+    (1e-10 relative, or 1e-13 of the mass for near-zero terms);
+  * at the benchmark sizes, under the developed cyclone flow with no velocity window,
+    mass(after) - mass(before) = limiter - outflow for h and A to 1e-13 of the mass in
+    every step, with both terms non-trivial over the run.
+"""
+import numpy as np
+import pytest
+
+from helpers import DEFAULT_PARAMS
+from oracle import Oracle, OracleDyn, OracleMesh
+
+pytestmark = pytest.mark.gpu
+
+ORA = Oracle()
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_spherical_mass_matches_oracle_meanvalue():
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    _say("spherical mass sums")
+    nx, ny = 40, 30
+    lon0, lon1, lat0, lat1 = -0.5, 0.4, 0.7, 1.2
+    coords = inputs.sphere_patch_coords(nx, ny, lon0, lon1, lat0, lat1)
+    mask = inputs.box_mask(nx, ny)
+    g = nd.MEVPDynamicsKernel(2, 6)
+    g.initialise(coords, True, mask, nx, ny)
+    om = OracleMesh.from_coords(nx, ny, coords, mask, spherical=True)
+    rng = np.random.default_rng(3)
+    h = rng.normal(1.0, 0.1, (nx * ny, 6))
+    A = rng.normal(0.9, 0.05, (nx * ny, 6))
+    g.setData("hice", h, 6)
+    g.setData("cice", A, 6)
+    vol, area = g.mass()
+    assert vol == pytest.approx(ORA.mean_value(om, 6, h), rel=1e-13)
+    assert area == pytest.approx(ORA.mean_value(om, 6, A), rel=1e-13)
+    g.setData("hice", np.ones(nx * ny), 1)
+    vol, _ = g.mass()
+    assert vol == pytest.approx((lon1 - lon0) * (np.sin(lat1) - np.sin(lat0)), rel=1e-12)
+
+
+def _module_pair(nx, seed):
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    s = inputs.benchmark_setup(nx, nx, seed)
+    o = OracleDyn(2, 6, dict(DEFAULT_PARAMS))
+    o.initialise(nx, nx, s["coords"], s["mask"])
+    g = nd.MEVPDynamicsKernel(2, 6)
+    g.initialise(s["coords"], False, s["mask"], nx, nx)
+    for name, key in (("uocean", "uocean"), ("vocean", "vocean")):
+        o.set_field(name, s[key], 1)
+        g.setData(name, s[key], 1)
+    return s, o, g
+
+
+def test_budget_tallies_match_oracle():
+    from nextsimdg_amd import inputs
+
+    _say("budget tallies against the oracle")
+    nx, dt = 32, 120.0
+    s, o, g = _module_pair(nx, seed=5)
+    g.set_budget(True)
+    oh, oA, gh, gA = s["h"], s["A"], s["h"], s["A"]
+    zero = np.zeros_like(s["h"])
+    for k in range(6):
+        uw, vw = ORA.benchmark_wind(nx, nx, inputs.L_BOX, k * dt, 0.0, 0.0, 60.0)
+        o.set_field("hice", oh, 1)
+        o.set_field("cice", oA, 1)
+        g.setData("hice", gh, 1)
+        g.setData("cice", gA, 1)
+        for name, v in (("uwind", uw), ("vwind", vw), ("ssh", zero)):
+            o.set_field(name, v, 1)
+            g.setData(name, v, 1)
+        o.update(dt)
+        g.update(dt)
+        oh, oA = o.get_field("hice", 1), o.get_field("cice", 1)
+        gh, gA = g.getDG0Data("hice"), g.getDG0Data("cice")
+    ob, gb = o.budget(), g.budget()
+    M = o.mass()
+    _say(f"  oracle {ob}\n  gpu    {gb}")
+    for key, m in (("outflow_h", M[0]), ("outflow_A", M[1]), ("limiter_h", M[0]), ("limiter_A", M[1])):
+        assert abs(gb[key] - ob[key]) <= max(1e-10 * abs(ob[key]), 1e-13 * m), key
+    assert ob["outflow_h"] > 1e-9 * M[0] and ob["limiter_A"] < -1e-9 * M[1]
+
+
+@pytest.mark.parametrize("nx", [512, 2048])
+def test_budget_closes_at_size(nx):
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    _say(f"budget closure at {nx}^2")
+    s = inputs.benchmark_setup(nx, nx, seed=2)
+    g = nd.MEVPDynamicsKernel(2, 6)
+    g.initialise(s["coords"], False, s["mask"], nx, nx)
+    for name, key in (("hice", "h"), ("cice", "A"), ("uocean", "uocean"), ("vocean", "vocean")):
+        g.setData(name, s[key], 1)
+    g.setData("ssh", 0 * s["h"], 1)
+    off = inputs.cyclone_offset(2)
+    g.set_budget(True)
+    tot = dict(outflow_h=0.0, outflow_A=0.0, limiter_h=0.0, limiter_A=0.0)
+    for k in range(6):
+        g.benchmark_wind(inputs.L_BOX, 120.0 * k, off[0], off[1], 60.0)
+        g.reset_dg0("hice")
+        g.reset_dg0("cice")
+        h0, a0 = g.mass()
+        g.update(120.0)
+        h1, a1 = g.mass()
+        b = g.budget(reset=True)
+        eh = abs((h1 - h0) - (b["limiter_h"] - b["outflow_h"])) / h0
+        ea = abs((a1 - a0) - (b["limiter_A"] - b["outflow_A"])) / a0
+        _say(f"  step {k}: dh={(h1 - h0) / h0:.3e} out={b['outflow_h'] / h0:.3e} lim={b['limiter_h'] / h0:.3e}"
+             f" closure {eh:.1e}; dA={(a1 - a0) / a0:.3e} out={b['outflow_A'] / a0:.3e}"
+             f" lim={b['limiter_A'] / a0:.3e} closure {ea:.1e}")
+        assert eh <= 1e-13 and ea <= 1e-13, (k, eh, ea)
+        for key in tot:
+            tot[key] += b[key]
+    assert tot["outflow_h"] > 1e-10 * h0 and tot["outflow_A"] > 1e-10 * a0, tot
+    assert tot["limiter_A"] < -1e-10 * a0, tot
+
+
+def test_general_mesh_node_count_limit_is_an_argument_error():
+    """The one-pass general-mesh kernels address CG nodes with 32-bit byte offsets, so a
+    non-uniform CG2 mesh with 2^29 nodes or more is refused (NSDG_ERR_ARG) at initialise."""
+    import nextsimdg_amd as nd
+
+    _say("node-count guard on a general mesh")
+    nx, ny = 16384, 8192  # (2 nx + 1)(2 ny + 1) = 536,920,065 > 2^29 CG2 nodes, 2^27 elements
+    j, i = np.meshgrid(np.arange(ny + 1, dtype=np.float64), np.arange(nx + 1, dtype=np.float64), indexing="ij")
+    coords = np.empty(((nx + 1) * (ny + 1), 2))
+    coords[:, 0] = (i * 100.0 + 3.0 * np.sin(j)).ravel()  # non-uniform: rows shifted unevenly
+    coords[:, 1] = (j * 100.0).ravel()
+    del i, j
+    g = nd.MEVPDynamicsKernel(2, 6)
+    with pytest.raises(nd.NsdgError) as err:
+        g.initialise(coords, False, np.ones(nx * ny), nx, ny)
+    assert err.value.code == -1 and "CG nodes" in str(err.value)  # NSDG_ERR_ARG
