@@ -205,6 +205,15 @@ int nsdg_dyn_reset_dg0(nsdg_dyn* d, const char* name);
  * spherical meshes (Tools.hpp:28-36; totals in radians squared there), accumulated with
  * compensated (two-sum) sums on the device in a fixed order. */
 int nsdg_dyn_mass(nsdg_dyn* d, double* ice_volume, double* ice_area);
+/* nsdg_dyn_mass without any allocation in the call. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The block partials and
+ * the two totals live in a fixed 1 MiB device array of the library (one per device, held
+ * until the process ends), so meshes of at most 8,386,560 elements are accepted; larger
+ * ones return NSDG_ERR_ARG. Results equal nsdg_dyn_mass bit for bit. Every call on a
+ * device shares that array: calls that overlap on one device (from several host threads)
+ * race. */
+int nsdg_dyn_mass_noalloc(nsdg_dyn* d, double* ice_volume, double* ice_area);
 /* Advection mass budget (no counterpart in the reference, which does not conserve mass
  * exactly: Dirichlet edges are outflow-only, DGTransport.cpp:306-351, and the limiters
  * clamp the cell mean, dgLimit.hpp:16-84). With the budget on, every update/advect adds,

@@ -423,6 +423,10 @@ int nsdg_dyn_mass(nsdg_dyn* d, double* vol, double* area)
 {
     return guard([&] { nonNull(d, "dyn")->k->mass(vol, area); });
 }
+int nsdg_dyn_mass_noalloc(nsdg_dyn* d, double* vol, double* area)
+{
+    return guard([&] { nonNull(d, "dyn")->k->massNoAlloc(vol, area); });
+}
 int nsdg_dyn_set_budget(nsdg_dyn* d, int on)
 {
     return guard([&] { nonNull(d, "dyn")->k->setBudget(on != 0); });

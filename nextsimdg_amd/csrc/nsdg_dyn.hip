@@ -23,6 +23,9 @@
 #include "nsdg_dyn.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 #include <cmath>
 #include <cstring>
 
@@ -2068,6 +2071,41 @@ __global__ __launch_bounds__(kBlock) void k_residual_nodes(NodeArgs a, const dou
     }
 }
 
+// ---------------------------------------------------------------- allocation-free sums
+// Scratch of nsdg_dyn_mass_noalloc: the two totals at [0, 2), block partials from
+// kNoAllocPartials on. 1 MiB per device, held until the process ends; every call on a
+// device shares it (k_mass_partial writes every partial it reads back and k_sum_final
+// writes both totals, so nothing needs clearing between calls; calls that overlap on one
+// device race).
+constexpr size_t kNoAllocScratch = size_t(1) << 17;
+constexpr size_t kNoAllocPartials = 32;
+constexpr size_t kNoAllocMaxElements = (kNoAllocScratch - kNoAllocPartials) / 4 * kBlock;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+// Device address of scratch on device dev (each device holds its own copy), looked up
+// once per device with that device current.
+static double* noAllocScratch(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, double*> ptrs;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end())
+        return it->second;
+    int cur = -1;
+    NSDG_HIP(hipGetDevice(&cur));
+    if (cur != dev)
+        NSDG_HIP(hipSetDevice(dev));
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+    if (cur != dev)
+        NSDG_HIP(hipSetDevice(cur));
+    NSDG_HIP(e);
+    require(p != nullptr, "no device address for the allocation-free scratch", NSDG_ERR_HIP);
+    ptrs[dev] = p;
+    return p;
+}
+
 // ---------------------------------------------------------------- learned warm start
 // Network input and target on the element grid of a CG2 kernel. Element e = ix + nx iy
 // owns the CG nodes (2 ix + a, 2 iy + b), a, b in {0, 1}, listed k = a + 2 b; its centre
@@ -2975,9 +3013,11 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     CG2DGOps cg2dgS;
     double hElem = 0.0;
 
+    int device = 0; // the device current at creation, which runs the stream's work
     explicit DynImpl(const nsdg_params& prm)
         : p(prm)
     {
+        NSDG_HIP(hipGetDevice(&device));
         NSDG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     }
     ~DynImpl() override
@@ -4058,6 +4098,29 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         checkLaunch("mass_final");
         double r[2];
         res.download(r, 2, s);
+        NSDG_HIP(hipStreamSynchronize(s));
+        if (vol)
+            *vol = r[0];
+        if (area)
+            *area = r[1];
+    }
+    // The sums of mass() with the block partials and totals in the file-scope scratch
+    // array instead of the kernel's own buffers: no allocation in the call.
+    void massNoAlloc(double* vol, double* area) override
+    {
+        need();
+        require(ne <= kNoAllocMaxElements,
+            "nsdg_dyn_mass_noalloc takes at most " + std::to_string(kNoAllocMaxElements) + " elements, the mesh has "
+                + std::to_string(ne),
+            NSDG_ERR_ARG);
+        double* const sp = noAllocScratch(device);
+        const unsigned nb = blocksFor(ne);
+        k_mass_partial<DG><<<nb, kBlock, 0, s>>>(view(mesh), static_cast<int>(ne), h.p, A.p, sp + kNoAllocPartials);
+        checkLaunch("mass_partial");
+        k_sum_final<2><<<1, kBlock, 0, s>>>(sp + kNoAllocPartials, static_cast<int>(nb), sp);
+        checkLaunch("mass_final");
+        double r[2];
+        NSDG_HIP(hipMemcpyAsync(r, sp, sizeof r, hipMemcpyDeviceToHost, s));
         NSDG_HIP(hipStreamSynchronize(s));
         if (vol)
             *vol = r[0];

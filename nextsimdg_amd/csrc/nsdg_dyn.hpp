@@ -30,6 +30,7 @@ struct DynBase {
     virtual void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) = 0;
     virtual void resetDG0(const std::string& name) = 0;
     virtual void mass(double* vol, double* area) = 0;
+    virtual void massNoAlloc(double* vol, double* area) = 0;
     virtual void setBudget(bool on) = 0;
     virtual void budget(double* out, bool reset) = 0;
     virtual void residual(double* r, double* normR, double* normD) = 0;

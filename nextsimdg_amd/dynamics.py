@@ -273,6 +273,13 @@ class MEVPDynamicsKernel:
         check(self.lib.nsdg_dyn_mass(self.h, ctypes.byref(v), ctypes.byref(a)))
         return v.value, a.value
 
+    def mass_noalloc(self):
+        """mass() with no allocation in the call (partial sums in the library's fixed
+        device array; at most 8,386,560 elements)."""
+        v, a = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.nsdg_dyn_mass_noalloc(self.h, ctypes.byref(v), ctypes.byref(a)))
+        return v.value, a.value
+
     def set_budget(self, on=True):
         """Switch the advection mass budget on (zeroed) or off."""
         check(self.lib.nsdg_dyn_set_budget(self.h, int(bool(on))))
