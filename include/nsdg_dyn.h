@@ -241,6 +241,20 @@ int nsdg_dyn_synchronize(nsdg_dyn* d);
  * library's own format. With buf == NULL, *size receives the required size. */
 int nsdg_dyn_save_state(nsdg_dyn* d, void* buf, size_t* size);
 int nsdg_dyn_load_state(nsdg_dyn* d, const void* buf, size_t size);
+/* Restart files in the layout of the reference's restart writer, ParaGridIO::dumpModelState
+ * (core/src/ParaGridIO.cpp:272-349): every ModelArray dimension (xdim, ydim, zdim, xvertex,
+ * yvertex, x_cg, y_cg, dg_comp, dgstress_comp, ncoords; ModelArrayDetails.cpp:28-41) and
+ * FP64 variables with a missing_value attribute, dimensions in the reference's order
+ * (components fastest): hice, cice (and damage) as (ydim, xdim, dg_comp); u, v as cell
+ * means (ydim, xdim); mask; coords (yvertex, xvertex, ncoords); plus the stress and strain
+ * s11..e22 (ydim, xdim, dgstress_comp) and the CG velocities u_cg, v_cg (and avgU_cg,
+ * avgV_cg) that restore this library's state exactly. The file is netCDF classic with
+ * 64-bit offsets (CDF-2), which has no groups: the reference's "data" group is the root
+ * and its metadata (structure type "parametric_rectangular", time) are global attributes.
+ * read_restart loads a file written for a kernel of the same mesh size, degrees and
+ * rheology (NSDG_ERR_ARG otherwise) and returns its time attribute. */
+int nsdg_dyn_write_restart(nsdg_dyn* d, const char* path, double time_seconds);
+int nsdg_dyn_read_restart(nsdg_dyn* d, const char* path, double* time_seconds);
 /* Per-kernel timing with HIP events on the library's stream. */
 int nsdg_dyn_set_profiling(nsdg_dyn* d, int on);
 int nsdg_dyn_kernel_stats(nsdg_dyn* d, const char* kernel, int* count, double* total_ms);

@@ -87,6 +87,8 @@ SIGNATURES = {
     "nsdg_dyn_mass": (_i, [_vp, _dp, _dp]),
     "nsdg_dyn_mass_noalloc": (_i, [_vp, _dp, _dp]),
     "nsdg_dyn_set_budget": (_i, [_vp, _i]),
+    "nsdg_dyn_write_restart": (_i, [_vp, ctypes.c_char_p, ctypes.c_double]),
+    "nsdg_dyn_read_restart": (_i, [_vp, ctypes.c_char_p, _dp]),
     "nsdg_dyn_budget": (_i, [_vp, _dp, _i]),
     "nsdg_dyn_residual": (_i, [_vp, _dp, _dp, _dp]),
     "nsdg_dyn_stress_from_velocity": (_i, [_vp]),

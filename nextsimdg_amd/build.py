@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnsdg_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["nsdg_mesh.hip", "nsdg_transport.hip", "nsdg_dyn.hip", "nsdg_cnn.hip", "nsdg_capi.hip"]
+SOURCES = ["nsdg_mesh.hip", "nsdg_transport.hip", "nsdg_dyn.hip", "nsdg_cnn.hip", "nsdg_capi.hip", "nsdg_restart.cpp"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [

@@ -427,6 +427,14 @@ int nsdg_dyn_mass_noalloc(nsdg_dyn* d, double* vol, double* area)
 {
     return guard([&] { nonNull(d, "dyn")->k->massNoAlloc(vol, area); });
 }
+int nsdg_dyn_write_restart(nsdg_dyn* d, const char* path, double time_seconds)
+{
+    return guard([&] { writeRestart(*nonNull(d, "dyn")->k, path, time_seconds); });
+}
+int nsdg_dyn_read_restart(nsdg_dyn* d, const char* path, double* time_seconds)
+{
+    return guard([&] { readRestart(*nonNull(d, "dyn")->k, path, time_seconds); });
+}
 int nsdg_dyn_set_budget(nsdg_dyn* d, int on)
 {
     return guard([&] { nonNull(d, "dyn")->k->setBudget(on != 0); });

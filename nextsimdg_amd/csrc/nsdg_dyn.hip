@@ -3028,6 +3028,15 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             (void)hipStreamDestroy(s);
     }
     Profiler& profiler() override { return prof; }
+    const HostMesh& meshHost() const override
+    {
+        need();
+        return mesh.host;
+    }
+    int cgDegree() const override { return CG; }
+    int dgComp() const override { return DG; }
+    int stressComp() const override { return S; }
+    bool brittleRheology() const override { return brittle; }
     bool initialised() const override { return ready; }
     size_t cgNodes() const override { return nn; }
     void need() const { require(ready, "kernel not initialised", NSDG_ERR_STATE); }

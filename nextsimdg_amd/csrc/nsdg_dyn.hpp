@@ -49,7 +49,18 @@ struct DynBase {
     virtual std::vector<char> saveState() = 0;
     virtual void loadState(const char* buf, size_t size) = 0;
     virtual Profiler& profiler() = 0;
+    // mesh and discretisation, for the restart files
+    virtual const HostMesh& meshHost() const = 0;
+    virtual int cgDegree() const = 0;
+    virtual int dgComp() const = 0;
+    virtual int stressComp() const = 0;
+    virtual bool brittleRheology() const = 0;
 };
+
+// Restart files (netCDF classic, 64-bit offsets) in the layout of ParaGridIO::dumpModelState
+// (core/src/ParaGridIO.cpp:272-349); nsdg_restart.cpp.
+void writeRestart(DynBase& k, const char* path, double timeSeconds);
+void readRestart(DynBase& k, const char* path, double* timeSeconds);
 
 std::unique_ptr<DynBase> makeDyn(int cg, int dg, const nsdg_params& p);
 

@@ -14,6 +14,7 @@ Arrays are numpy float64 in the ModelArray layout (element ix + nx*iy, component
 contiguous). Errors raise NsdgError (the reference aborts or throws).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -279,6 +280,17 @@ class MEVPDynamicsKernel:
         v, a = ctypes.c_double(), ctypes.c_double()
         check(self.lib.nsdg_dyn_mass_noalloc(self.h, ctypes.byref(v), ctypes.byref(a)))
         return v.value, a.value
+
+    def write_restart(self, path, time_seconds=0.0):
+        """Write the state as a restart file in the layout of ParaGridIO::dumpModelState
+        (netCDF classic, 64-bit offsets; see include/nsdg_dyn.h)."""
+        check(self.lib.nsdg_dyn_write_restart(self.h, os.fsencode(path), float(time_seconds)))
+
+    def read_restart(self, path):
+        """Load a restart file written for a kernel like this one; returns its time."""
+        t = ctypes.c_double()
+        check(self.lib.nsdg_dyn_read_restart(self.h, os.fsencode(path), ctypes.byref(t)))
+        return t.value
 
     def set_budget(self, on=True):
         """Switch the advection mass budget on (zeroed) or off."""
