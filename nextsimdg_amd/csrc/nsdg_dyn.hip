@@ -1491,7 +1491,11 @@ struct MevpNodeWindow {
         momentumFrozen(a, in, dx, dy, un, vn);
     }
 };
-template <int CG, int DG>
+// R = float: the FP32 variant, two of k_mevp_fused<CG, DG, float>'s subcycles in one
+// pass (the stored stress of each sub-step stochastically rounded with the subcycle
+// indices iter and iter + 1, as two single passes round it; the divergence and sub-step 2
+// use the rounded values; the node update stays FP64).
+template <int CG, int DG, class R>
 __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
 {
     using T = PairTile<CG>;
@@ -1499,8 +1503,8 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
     constexpr int NWH = T::NW * T::NH, NB = (NWH + kBlock - 1) / kBlock;
     // the sub-step contributions, and sigma^(k+1) parked during sub-step 2, in one array
     constexpr int CWORDS = T::W1 * T::H1 * N * 2, SWORDS = 3 * S * T::W2 * T::H2;
-    __shared__ double contribRaw[CWORDS > SWORDS ? CWORDS : SWORDS];
-    auto& contrib = *reinterpret_cast<double(*)[T::W1 * T::H1][N][2]>(contribRaw);
+    __shared__ R contribRaw[CWORDS > SWORDS ? CWORDS : SWORDS];
+    auto& contrib = *reinterpret_cast<R(*)[T::W1 * T::H1][N][2]>(contribRaw);
     __shared__ double uS[NWH], vS[NWH];
     const NodeArgs& a = f.n;
     const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
@@ -1514,23 +1518,23 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
     const int ex = ex0 - 2 + lx, ey = ey0 - 2 + ly;
     const bool inDomain = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
     const int e = inDomain ? ex + a.nx * ey : 0;
-    double s11[S], s12[S], s22[S]; // sigma^(k+1) of this thread's element
+    R s11[S], s12[S], s22[S]; // sigma^(k+1) of this thread's element
     {
-        double tx[N], ty[N];
+        R tx[N], ty[N];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            tx[j] = ty[j] = 0.0;
+            tx[j] = ty[j] = 0;
         if (inDomain) {
-            Acc<double> acc1[S], acc2[S], acc3[S];
-            elementAffineCompute<CG, DG, double, double, false>(f.e, e, acc1, acc2, acc3);
+            Acc<R> acc1[S], acc2[S], acc3[S];
+            elementAffineCompute<CG, DG, R, R, false>(f.e, e, acc1, acc2, acc3);
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                s11[k] = acc1[k].get();
-                s12[k] = acc2[k].get();
-                s22[k] = acc3[k].get();
+                s11[k] = storeValue<R>(acc1[k], e, 0 * S + k, f.e.iter);
+                s12[k] = storeValue<R>(acc2[k], e, 1 * S + k, f.e.iter);
+                s22[k] = storeValue<R>(acc3[k], e, 2 * S + k, f.e.iter);
             }
             if (a.ocean[e])
-                divergenceAffineFrom<CG, double>(a, s11, s12, s22, tx, ty);
+                divergenceAffineFrom<CG, R>(a, s11, s12, s22, tx, ty);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -1564,8 +1568,8 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
                 double dx = 0.0, dy = 0.0;
                 forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int exa, int eya, int j) {
                     const int le = (exa - ex0 + 2) + T::W1 * (eya - ey0 + 2);
-                    dx -= contrib[le][j][0];
-                    dy -= contrib[le][j][1];
+                    dx -= static_cast<double>(contrib[le][j][0]);
+                    dy -= static_cast<double>(contrib[le][j][1]);
                 });
                 momentumFrozen(a, in[b], dx, dy, un, vn);
             }
@@ -1577,7 +1581,7 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
     // sub-step 2, the inner elements, from sigma^(k+1) parked in LDS and the LDS velocities
     {
         constexpr int NI = T::W2 * T::H2;
-        double* const sigS = contribRaw; // 3 S NI <= SWORDS values
+        R* const sigS = contribRaw; // 3 S NI <= SWORDS values
         const bool inner = lx >= 1 && lx <= T::W2 && ly >= 1 && ly <= T::H2;
         const int le2 = (lx - 1) + T::W2 * (ly - 1);
         const bool active = inner && inDomain;
@@ -1589,14 +1593,14 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
                 sigS[(2 * S + k) * NI + le2] = s22[k];
             }
         }
-        double tx[N], ty[N];
+        R tx[N], ty[N];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            tx[j] = ty[j] = 0.0;
+            tx[j] = ty[j] = 0;
         if (active) {
             const int bx = CG * (ex - ex0 + 1), by = CG * (ey - ey0 + 1); // first node in the window
-            Acc<double> acc1[S], acc2[S], acc3[S];
-            elementAffineCore<CG, DG, double, double, false>(
+            Acc<R> acc1[S], acc2[S], acc3[S];
+            elementAffineCore<CG, DG, R, R, false>(
                 f.e, e, acc1, acc2, acc3,
                 [&](double* lu, double* lv) {
 #pragma unroll
@@ -1607,18 +1611,22 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
                             lv[c + K * r] = vS[bx + c + T::NW * (by + r)];
                         }
                 },
-                [&](int c, int k) -> double { return sigS[(c * S + k) * NI + le2]; });
-            double n11[S], n12[S], n22[S];
+                [&](int c, int k) -> R { return sigS[(c * S + k) * NI + le2]; });
+            R n11[S], n12[S], n22[S];
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                n11[k] = acc1[k].get();
-                n12[k] = acc2[k].get();
-                n22[k] = acc3[k].get();
+                n11[k] = storeValue<R>(acc1[k], e, 0 * S + k, f.e.iter + 1);
+                n12[k] = storeValue<R>(acc2[k], e, 1 * S + k, f.e.iter + 1);
+                n22[k] = storeValue<R>(acc3[k], e, 2 * S + k, f.e.iter + 1);
             }
-            if (lx >= 2 && ly >= 2)
-                storeTensor<S>(f.sigOut, a.ne, e, n11, n12, n22);
+            if (lx >= 2 && ly >= 2) {
+                if constexpr (sizeof(R) == 8)
+                    storeTensor<S>(f.sigOut, a.ne, e, n11, n12, n22);
+                else
+                    storeTensor<S>(f.sigFOut, a.ne, e, n11, n12, n22);
+            }
             if (a.ocean[e])
-                divergenceAffineFrom<CG, double>(a, n11, n12, n22, tx, ty);
+                divergenceAffineFrom<CG, R>(a, n11, n12, n22, tx, ty);
         }
         __syncthreads(); // every parked stress is read before the contributions overwrite it
         if (inner) {
@@ -3596,12 +3604,12 @@ template <int CG, int DG> struct DynImpl final : DynBase {
             k_strain_geom<CG, false><<<blocksFor(ne), kBlock, 0, s>>>(a, geo.p);
         checkLaunch("strain");
     }
-    // Two subcycles per pass: uniform FP64 meshes without periodic segments, by default
+    // Two subcycles per pass: uniform meshes without periodic segments (FP64 or FP32), by default
     // from NSDG_PAIR_MIN_ELEMENTS up, where it is faster per subcycle than k_mevp_fused
     // (below, the state largely stays in the memory-side cache).
     bool pairSubcycles() const
     {
-        const bool can = fusedSubcycle() && mesh.uniform && precision == NSDG_PREC_FP64;
+        const bool can = fusedSubcycle() && mesh.uniform;
         return can && (pairing == NSDG_PAIR_ALWAYS || (pairing == NSDG_PAIR_AUTO && ne >= NSDG_PAIR_MIN_ELEMENTS));
     }
     void pairPass()
@@ -3611,13 +3619,22 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         f.e = elemArgs();
         f.n = nodeArgs();
         f.sigOut = sigAlt.p;
+        f.sigFOut = sigFAlt.p;
         f.uOut = uAlt.p;
         f.vOut = vAlt.p;
         using T = PairTile<CG>;
         const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
-        k_mevp_pair<CG, DG><<<grid, kBlock, 0, s>>>(f);
+        if (precision == NSDG_PREC_FP32) {
+            require(sigF.n == 3 * S * ne && sigFAlt.n == sigF.n, "FP32 buffers not allocated", NSDG_ERR_STATE);
+            k_mevp_pair<CG, DG, float><<<grid, kBlock, 0, s>>>(f);
+        } else {
+            k_mevp_pair<CG, DG, double><<<grid, kBlock, 0, s>>>(f);
+        }
         checkLaunch("mevp_pair");
-        swapBuf(sig, sigAlt);
+        if (precision == NSDG_PREC_FP32)
+            swapBuf(sigF, sigFAlt);
+        else
+            swapBuf(sig, sigAlt);
         swapBuf(u, uAlt);
         swapBuf(v, vAlt);
     }

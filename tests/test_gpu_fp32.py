@@ -107,3 +107,61 @@ def test_fp32_needs_a_uniform_mesh():
     k.begin_iteration(120.0)
     k.subcycles(2)
     k.synchronize()
+
+
+@pytest.mark.parametrize("nx,ny", [(2, 2), (13, 13), (27, 27), (40, 29)])
+def test_fp32_paired_subcycles_equal_single_passes(nx, ny):
+    """The FP32 variant with two subcycles per pass (k_mevp_pair<float>, forced on) against
+    one per pass (k_mevp_fused<float>) from the same state, nine subcycles. The pair
+    kernel rounds each sub-step's stored stress stochastically with the same subcycle
+    indices as two single passes, so the two agree except where the compiler fuses a
+    multiply-add differently and a stochastic rounding flips by one FP32 ulp (6e-8
+    relative). Bound: 1e-6 relative (max norm) per field; a second pair run gives the same
+    bits; and the result is FP32 arithmetic (it differs from FP64 by more than 1e-10)."""
+    import nextsimdg_amd as nd
+    from helpers import box_coords
+
+    _say(f"fp32 pairs vs single passes {nx}x{ny}")
+    cg, dg, S = 2, 6, 8
+    coords = box_coords(nx, ny, 2000.0)
+    mask = np.ones(nx * ny)
+    if nx * ny >= 6:
+        mask[(ny // 2) * nx + nx // 2] = 0.0
+    ne, ncg = nx * ny, (cg * nx + 1) * (cg * ny + 1)
+    rng = np.random.default_rng(nx * 1000 + ny)
+    state = {"hice": np.abs(rng.normal(1.0, 0.3, (ne, dg))) * np.r_[1.0, [0.05] * (dg - 1)],
+             "cice": np.clip(rng.normal(0.9, 0.05, (ne, dg)), 0, 1) * np.r_[1.0, [0.02] * (dg - 1)]}
+    for n in ("s11", "s12", "s22", "e11", "e12", "e22"):
+        state[n] = rng.normal(0, 1e3 if n[0] == "s" else 1e-7, (ne, S))
+    forcing = {n: rng.normal(0, sc, ne) for n, sc in (("uwind", 10), ("vwind", 10), ("uocean", 0.05),
+                                                       ("vocean", 0.05), ("ssh", 0.1))}
+    vel = {n: rng.normal(0, 0.1, ncg) for n in ("u", "v")}
+    out = []
+    for prec, mode in ((FP32, 1), (FP32, 1), (FP32, 2), (FP64, 2)):
+        k = nd.MEVPDynamicsKernel(cg, dg)
+        k.initialise(coords, False, mask, nx, ny)
+        k.set_pairing(mode)
+        for n, v in state.items():
+            k.setData(n, v, v.shape[1])
+        for n, v in forcing.items():
+            k.setData(n, v, 1)
+        for n, v in vel.items():
+            k.set_cg(n, v)
+        k.set_precision(prec)
+        k.prepareIteration()
+        k.begin_iteration(120.0)
+        k.subcycles(9)
+        k.synchronize()
+        f = {n: k.get_cg(n) for n in ("u", "v")}
+        f.update({n: k.getDGData(n, S) for n in ("s11", "s12", "s22")})
+        out.append(f)
+    a, a2, b, d64 = out
+    assert np.abs(a["u"]).max() > 1e-3
+    errs = {}
+    for n in a:
+        assert np.array_equal(a[n], a2[n]), n
+        errs[n] = np.abs(a[n] - b[n]).max() / max(np.abs(b[n]).max(), 1e-300)
+    _say("  pair vs single: " + " ".join(f"{n}={e:.1e}" for n, e in errs.items()))
+    for n, e in errs.items():
+        assert e <= 1e-6, (n, e)
+    assert max(rel_l2(a[n], d64[n]) for n in a) > 1e-10
