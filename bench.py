@@ -41,6 +41,15 @@ def byte_model(nx, ny, cg=2, we=8):
     return elem, node, fused
 
 
+def survey_byte_model(nx, ny, cg=2, dg=6, we=8):
+    """Algorithmic bytes of one mEVP subcycle of the reference's two-pass formulation (the
+    roofline figure): element pass u, v at the nodes, h and A, stress read and written;
+    node pass the stress gathered and 13 node fields read, u and v written:
+    B = we (9 S + 2 D) N_e + 8 * 17 N_n (we = 8 for FP64 element data, 4 for FP32)."""
+    S = 8 if cg == 2 else 3
+    return we * (9 * S + 2 * dg) * nx * ny + 8 * 17 * (cg * nx + 1) * (cg * ny + 1)
+
+
 def make_kernel(nx, ny, seed, prec, kind="plain", perturb=0.0, brittle=False, warp=0.0):
     """The benchmark box; warp > 0 shifts the vertices by warp * sin(y / 9 km) in x, which
     makes every element a non-affine quadrilateral (the general-mesh kernels)."""
@@ -300,9 +309,28 @@ def learned_vs_plain(nx, kinds=("heterogeneous", "leads", "plain"), seed=5000, s
     return out
 
 
-def cpu_baseline(nx, ny, seed=0, target_s=12.0):
-    """The FP64 oracle (OpenMP C++ restatement of the reference) on the host cores:
-    mEVP subcycles of the same box, timed until about target_s seconds."""
+def cpu_info():
+    """CPU model, logical CPUs of the host, and the CPUs this process may run on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def _oracle_threads():
+    """All the cores this run may use: OMP_NUM_THREADS when set, else the CPUs available."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env and env.isdigit() else cpu_info()["cpus_available"]
+
+
+def _oracle_subcycles(nx, ny, seed, target_s):
+    """mEVP subcycles of the config-2 box on the FP64 oracle for about target_s seconds."""
     from nextsimdg_amd import inputs
     from oracle import Oracle, OracleDyn
 
@@ -318,19 +346,194 @@ def cpu_baseline(nx, ny, seed=0, target_s=12.0):
     o.begin_iteration(120.0)
     o.run_pass("subcycle", 1)  # warm
     n, t = 0, time.perf_counter()
-    while time.perf_counter() - t < target_s:
+    while time.perf_counter() - t < target_s or n < 2:
         o.run_pass("subcycle", 1)
         n += 1
-    el = time.perf_counter() - t
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
+    return n, time.perf_counter() - t
+
+
+def cpu_baseline(nx, ny, seed=0, target_s=10.0):
+    """The FP64 oracle (OpenMP C++ restatement of the reference) on the host cores: mEVP
+    subcycles of the same box for about target_s seconds, on all cores and on one."""
+    import oracle
+
+    threads = _oracle_threads()
+    oracle.set_threads(threads)
+    n, el = _oracle_subcycles(nx, ny, seed, target_s)
+    oracle.set_threads(1)
+    n1, el1 = _oracle_subcycles(nx, ny, seed, target_s)
+    oracle.set_threads(threads)
+    return dict({
         "value": round(nx * ny * n / el / 1e6, 4),
         "unit": "Mcell-mEVP-iters/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} mEVP subcycles of the {nx}x{ny} CG2/DG2 box on the FP64 oracle "
-                  f"(advection excluded), {el:.1f} s, OMP_NUM_THREADS={threads}",
-    }
+        "sample": f"{n} mEVP subcycles of the {nx}x{ny} CG2/DG2 box on the FP64 oracle (advection excluded), "
+                  f"{el:.1f} s on {threads} threads",
+        "one_thread": {"value": round(nx * ny * n1 / el1 / 1e6, 4), "unit": "Mcell-mEVP-iters/s", "cores": 1,
+                       "sample": f"{n1} subcycles, {el1:.1f} s"},
+    }, **cpu_info())
+
+
+def _config1_steps(k, setf, getf, nx, winds, h, A, t_limit=None):
+    """Module-level steps of config 1 (DG0 h and A, forcing, update); stops early after
+    t_limit seconds. Returns (steps run, seconds)."""
+    import numpy as np
+
+    zero = np.zeros(nx * nx)
+    t = time.perf_counter()
+    n = 0
+    for uw, vw in winds:
+        setf("hice", h, 1)
+        setf("cice", A, 1)
+        setf("uwind", uw, 1)
+        setf("vwind", vw, 1)
+        setf("ssh", zero, 1)
+        k.update(120.0)
+        h, A = getf("hice"), getf("cice")
+        n += 1
+        if t_limit and time.perf_counter() - t > t_limit:
+            break
+    if hasattr(k, "synchronize"):
+        k.synchronize()
+    return n, time.perf_counter() - t
+
+
+def config1(nx=128, steps=720, cpu_window_s=10.0):
+    """Config 1, the plain method on the reference's CPU path: 128 x 128 box (dx = 4 km),
+    CG1 velocity, DG1 advection, 3-dof stress, dt = 120 s, 100 subcycles, 24 h (720 steps)
+    of the moving cyclone. The FP64 oracle runs a window of the 720 steps (about
+    cpu_window_s seconds) on all cores and on one; the GPU runs all 720 steps. Module-level
+    steps both sides (host round trips of h, A and the forcing included)."""
+    import numpy as np
+    import oracle
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    s = inputs.benchmark_setup(nx, nx, 0)
+    off = inputs.cyclone_offset(0)
+    ora = oracle.Oracle()
+    winds = [ora.benchmark_wind(nx, nx, inputs.L_BOX, i * 120.0, off[0], off[1], 30.0) for i in range(steps)]
+    out = {"workload": f"config 1: {nx}x{nx} box, dx=4 km, CG1/DG1, 3-dof stress, FP64, dt=120 s, 100 subcycles, "
+                       f"moving cyclone, {steps} steps (24 h), module-level steps with host I/O",
+           "unit": "Mcell-mEVP-iters/s"}
+    g = nd.MEVPDynamicsKernel(1, 3)
+    g.initialise(s["coords"], False, s["mask"], nx, nx)
+    for name in ("uocean", "vocean"):
+        g.setData(name, s[name], 1)
+    _config1_steps(g, g.setData, g.getDG0Data, nx, winds[:5], s["h"], s["A"])  # warm
+    n, el = _config1_steps(g, g.setData, g.getDG0Data, nx, winds, s["h"], s["A"])
+    out["gpu"] = {"steps": n, "seconds": round(el, 3), "value": round(nx * nx * 100 * n / el / 1e6, 3)}
+    threads = _oracle_threads()
+    for tag, th, lim in (("cpu_all_cores", threads, cpu_window_s), ("cpu_one_thread", 1, cpu_window_s)):
+        oracle.set_threads(th)
+        o = oracle.OracleDyn(1, 3)
+        o.initialise(nx, nx, s["coords"], s["mask"])
+        for name in ("uocean", "vocean"):
+            o.set_field(name, s[name], 1)
+        n, el = _config1_steps(o, o.set_field, lambda name: o.get_field(name, 1), nx, winds, s["h"], s["A"], lim)
+        out[tag] = {"threads": th, "steps": n, "seconds": round(el, 3),
+                    "value": round(nx * nx * 100 * n / el / 1e6, 4),
+                    "projected_24h_seconds": round(el / n * steps, 1)}
+    oracle.set_threads(threads)
+    out["gpu_vs_cpu_all_cores"] = round(out["gpu"]["value"] / out["cpu_all_cores"]["value"], 1)
+    out.update(cpu_info())
+    del np
+    return out
+
+
+def budget_leg(nx, steps, seed=0):
+    """The advection mass budget of the benchmark box over `steps` module-level steps:
+    change of the ice volume and area (Tools::MeanValue) against the Dirichlet outflow and
+    the limiters' change, relative to the starting mass; closure = |change - (limiter -
+    outflow)| / mass, the worst step."""
+    k, off, L = make_kernel(nx, nx, seed, 0)
+    k.set_budget(True)
+    m0 = k.mass()
+    worst = [0.0, 0.0]
+    tot = dict(outflow_h=0.0, outflow_A=0.0, limiter_h=0.0, limiter_A=0.0)
+    for i in range(steps):
+        k.reset_dg0("hice")
+        k.reset_dg0("cice")
+        k.benchmark_wind(L, i * 120.0, off[0], off[1], 30.0)
+        a = k.mass()
+        k.update(120.0)
+        b = k.mass()
+        bud = k.budget(reset=True)
+        for f, key in ((0, "h"), (1, "A")):
+            err = abs((b[f] - a[f]) - (bud["limiter_" + key] - bud["outflow_" + key])) / a[f]
+            worst[f] = max(worst[f], err)
+        for key in tot:
+            tot[key] += bud[key]
+    m1 = k.mass()
+    out = {"workload": f"benchmark box {nx}x{nx}, CG2/DG2 FP64, {steps} module-level steps of the moving cyclone"}
+    for f, key, name in ((0, "h", "volume"), (1, "A", "area")):
+        out[name] = {"rel_change": float(f"{(m1[f] - m0[f]) / m0[f]:.6e}"),
+                     "rel_outflow": float(f"{tot['outflow_' + key] / m0[f]:.6e}"),
+                     "rel_limiter": float(f"{tot['limiter_' + key] / m0[f]:.6e}"),
+                     "closure_worst_step": float(f"{worst[f]:.3e}")}
+    return out
+
+
+def config5_leg(nx, seed=5, spinup=2, check_every=25):
+    """Config 5: 2048 x 2048 box, three seeded storms (10-25 m/s), heterogeneous ice with
+    open-water leads. One timestep after `spinup` plain steps, from one saved state: plain
+    FP64 mEVP (100 subcycles) against FP32 with the gated learned start iterating until the
+    momentum residual reaches plain's: subcycles to the residual and wall time."""
+    import numpy as np
+    from nextsimdg_amd import inputs
+    from nextsimdg_amd import learned as Lrn
+
+    st = inputs.storms(seed)
+    k, _, L = make_kernel(nx, nx, seed, 0, "leads")
+    for i in range(spinup):
+        k.reset_dg0("hice")
+        k.reset_dg0("cice")
+        k.storm_wind(L, i * 120.0, st)
+        k.update(120.0)
+    k.reset_dg0("hice")
+    k.reset_dg0("cice")
+    k.storm_wind(L, spinup * 120.0, st)
+    k.advectionAndLimits(120.0)
+    k.prepareIteration()
+    k.begin_iteration(120.0)
+    blob = k.save_state()
+    r_cold = k.momentum_residual()[0]
+    k.subcycles(100)  # warm
+    k.load_state(blob)
+    k.synchronize()
+    t = time.perf_counter()
+    k.subcycles(100)
+    k.synchronize()
+    t_plain = time.perf_counter() - t
+    r_plain = k.momentum_residual()[0]
+    out = {"workload": f"config 5: {nx}x{nx} CG2/DG2 box, 3 seeded storms (peaks "
+                       f"{', '.join(f'{v / np.e:.1f}' for v in st[:, 2])} m/s), heterogeneous ice with open-water "
+                       f"leads, one timestep after {spinup} plain steps",
+           "r_cold": float(f"{r_cold:.5e}"), "r_plain_100": float(f"{r_plain:.5e}"),
+           "t_plain_ms": round(t_plain * 1e3, 3)}
+    net = None
+    if os.path.exists(WEIGHTS):
+        net = Lrn.WarmStartNet(0)
+        net.set_params(np.load(WEIGHTS))
+    for tag, learned in (("fp32", False), ("fp32_learned", True)):
+        if learned and net is None:
+            continue
+        k.load_state(blob)
+        k.set_precision(1)
+        k.synchronize()
+        t = time.perf_counter()
+        rec = {}
+        if learned:
+            rp, rc, used = Lrn.warm_start(k, net)
+            rec.update(r_predicted=float(f"{rp:.5e}"), prediction_kept=used)
+        done, r = Lrn.subcycles_to_residual(k, r_plain, 1000, check_every)
+        el = time.perf_counter() - t
+        rec.update(subcycles_to_r_plain=done, r_reached=float(f"{r:.5e}"), t_ms=round(el * 1e3, 3),
+                   speedup_vs_plain_fp64=round(t_plain / el, 3))
+        out[tag] = rec
+        k.set_precision(0)
+    return out
 
 
 def calibrate_copy(mib=1024, reps=3):
@@ -364,6 +567,9 @@ def main():
     ap.add_argument("--bbm-general-nx", type=int, default=512,
                     help="brittle step on a non-affine mesh of this size (0 skips it)")
     ap.add_argument("--spherical-nx", type=int, default=512, help="spherical patch size (0 skips it)")
+    ap.add_argument("--budget-steps", type=int, default=10, help="mass budget steps at --nx (0 skips it)")
+    ap.add_argument("--config5-nx", type=int, default=2048, help="config 5 (storms, leads) size (0 skips it)")
+    ap.add_argument("--no-config1", action="store_true", help="skip config 1 (128^2 CG1/DG1, CPU and GPU)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -390,13 +596,14 @@ def main():
         _, rstats = measure(rnx, rnx, 1, 1)
     else:
         rstats = stats
-    alg_of = dict(zip(("mevp_element", "mevp_node", "mevp_fused"), byte_model(rnx, rnx)))
-    # two subcycles per launch: the algorithmic bytes of both (the state is read and written
-    # once per pair, which is what the pair saves)
-    alg_of["mevp_pair"] = 2 * alg_of["mevp_fused"]
-    dom = max(alg_of, key=lambda n: rstats[n][1])
+    fused_of = dict(zip(("mevp_element", "mevp_node", "mevp_fused"), byte_model(rnx, rnx)))
+    fused_of["mevp_pair"] = 2 * fused_of["mevp_fused"]
+    # subcycles per launch of each kernel: the pair kernel runs two (reading and writing the
+    # state once per two subcycles is what it saves)
+    per_launch = {"mevp_element": 1, "mevp_node": 1, "mevp_fused": 1, "mevp_pair": 2}
+    dom = max(per_launch, key=lambda n: rstats[n][1])
     avg_ms = rstats[dom][1] / rstats[dom][0]
-    alg = alg_of[dom]
+    alg = per_launch[dom] * survey_byte_model(rnx, rnx)
     achieved = alg / (avg_ms * 1e-3)
     traffic = _traffic(args.traffic, f"{dom}_{rnx}")
     # % of HBM peak on the two-pass byte model of the reference's formulation (element pass:
@@ -422,8 +629,17 @@ def main():
                    "nx": nx, "ny": ny, "subcycles": 100, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "grid": f"{rnx}x{rnx}", "achieved": round(achieved / 1e9, 1),
                      "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 5),
-                     "launches": rstats[dom][0]},
+                     "algorithmic_bytes_per_launch": alg,
+                     "byte_model": "8 (9 S + 2 D) N_e + 8 * 17 N_n per subcycle x subcycles per launch "
+                                   f"({per_launch[dom]})",
+                     "avg_launch_ms": round(avg_ms, 5), "launches": rstats[dom][0],
+                     # the one-pass kernels' own compulsory bytes (stress r+w, strength, 10 node
+                     # values): a tighter model than the two-pass one above
+                     "achieved_fused_model": round(fused_of[dom] / (avg_ms * 1e-3) / 1e9, 1),
+                     "frac_fused_model": round(fused_of[dom] / (avg_ms * 1e-3) / HBM_PEAK, 4),
+                     # HBM bytes the counters measured per launch, over the launch time
+                     "counter_rate": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "counter_frac": round(traffic / (avg_ms * 1e-3) / HBM_PEAK, 4) if traffic else None},
         "kernels_ms": {n: (round(v[1], 5) if v[1] else None) for n, v in ms.items()},
         "hbm_peak_frac": round(value * 1e6 / ne * ref_bytes / HBM_PEAK, 4),
         "hbm_bytes_per_subcycle_reference_model": ref_bytes,
@@ -447,12 +663,24 @@ def main():
     if args.spherical_nx:
         log(f"bench: spherical patch at {args.spherical_nx}^2 ...")
         out["spherical"] = spherical_leg(args.spherical_nx, max(2, args.steps // 2), 1, traffic_path=args.traffic)
+    if args.budget_steps:
+        log(f"bench: mass budget at {nx}^2 ...")
+        out["budget"] = budget_leg(nx, args.budget_steps)
+    if args.config5_nx:
+        log(f"bench: config 5 (storms, leads) at {args.config5_nx}^2 ...")
+        out["config5"] = config5_leg(args.config5_nx)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:
             out["cpu_baseline"] = cpu_baseline(nx, ny)
         except Exception as e:  # noqa: BLE001 - the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        if not args.no_config1:
+            log("bench: config 1 (128^2 CG1/DG1, oracle and GPU) ...")
+            try:
+                out["config1"] = config1()
+            except Exception as e:  # noqa: BLE001
+                out["config1"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 

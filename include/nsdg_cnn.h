@@ -78,7 +78,9 @@ int nsdg_dyn_apply_velocity(nsdg_dyn* d, const float* y);
  * residual gate: when r(predicted) > r(cold start: u0 and the previous stress), the cold
  * start is restored. *used = 1 when the prediction was kept. */
 int nsdg_dyn_warm_start(nsdg_dyn* d, nsdg_cnn* net, double* r_predicted, double* r_cold, int* used);
-/* Subcycles in chunks of check_every until r <= r_target or max_subcycles have run. */
+/* Subcycles until r <= r_target or max_subcycles have run, in chunks of at most
+ * check_every: once r has fallen over a chunk, the next chunk ends where its rate of
+ * decrease reaches r_target (at least one subcycle). */
 int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, int* done,
     double* r_final);
 

@@ -197,6 +197,11 @@ int nsdg_dyn_set_pairing(nsdg_dyn* d, int mode);
  * with the centre shifted by (xoff, yoff) metres and the given vMax, then set as the
  * DG0 uwind/vwind fields (nodal averages, as setData does). */
 int nsdg_dyn_benchmark_wind(nsdg_dyn* d, double L, double t_seconds, double xoff, double yoff, double vmax);
+/* Several storms (no counterpart in the reference, which has one): the sum of n <= 8
+ * copies of that cyclone on its track, storm i shifted by (xoff[i], yoff[i]) metres with
+ * vMax vmax[i] (its peak speed is vmax / e, at 100 km from the centre). */
+int nsdg_dyn_storm_wind(nsdg_dyn* d, double L, double t_seconds, const double* xoff, const double* yoff,
+    const double* vmax, int n);
 /* Zero the higher DG moments of a field on the device: the effect of the module-level
  * setData(name, getDG0Data(name)) round trip of MEVPDynamics::update (MEVPDynamics.cpp:64-65). */
 int nsdg_dyn_reset_dg0(nsdg_dyn* d, const char* name);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "nsdg_dyn_set_precision": (_i, [_vp, _i]),
     "nsdg_dyn_set_pairing": (_i, [_vp, _i]),
     "nsdg_dyn_benchmark_wind": (_i, [_vp, _d, _d, _d, _d, _d]),
+    "nsdg_dyn_storm_wind": (_i, [_vp, _d, _d, _dp, _dp, _dp, _i]),
     "nsdg_dyn_reset_dg0": (_i, [_vp, _s]),
     "nsdg_dyn_mass": (_i, [_vp, _dp, _dp]),
     "nsdg_dyn_mass_noalloc": (_i, [_vp, _dp, _dp]),

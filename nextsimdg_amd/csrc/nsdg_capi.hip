@@ -415,6 +415,11 @@ int nsdg_dyn_benchmark_wind(nsdg_dyn* d, double L, double t, double xoff, double
 {
     return guard([&] { nonNull(d, "dyn")->k->benchmarkWind(L, t, xoff, yoff, vmax); });
 }
+int nsdg_dyn_storm_wind(nsdg_dyn* d, double L, double t, const double* xoff, const double* yoff, const double* vmax,
+    int n)
+{
+    return guard([&] { nonNull(d, "dyn")->k->stormWind(L, t, xoff, yoff, vmax, n); });
+}
 int nsdg_dyn_reset_dg0(nsdg_dyn* d, const char* name)
 {
     return guard([&] { nonNull(d, "dyn")->k->resetDG0(nonNull(name, "name")); });

@@ -2221,9 +2221,14 @@ __global__ void k_ice_ocean_stress(int nn, double F_ocean, const double* u, cons
     tx[n] = F_ocean * a * ur;
     ty[n] = F_ocean * a * vr;
 }
-// Benchmark cyclone (BenchmarkAtmosphere.cpp:29-76) at the cell lower-left corners.
-__global__ void k_benchmark_wind(int nx, int ny, double dx, double dy, double x0, double y0, double vmax, double* uw,
-    double* vw)
+// Benchmark cyclone (BenchmarkAtmosphere.cpp:29-76) at the cell lower-left corners; with
+// several storms, the sum of copies of it, storm i centred at (x0[i], y0[i]) with its vMax.
+constexpr int kMaxStorms = 8;
+struct Storms {
+    int n;
+    double x0[kMaxStorms], y0[kMaxStorms], vmax[kMaxStorms];
+};
+__global__ void k_benchmark_wind(int nx, int ny, double dx, double dy, Storms st, double* uw, double* vw)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nx * ny)
@@ -2231,10 +2236,15 @@ __global__ void k_benchmark_wind(int nx, int ny, double dx, double dy, double x0
     const double A = 1e-5, k = 1e-5;
     const double alpha = 72. / 180. * M_PI;
     const double ca = cos(alpha), sa = sin(alpha);
-    const double xp = (e % nx) * dx - x0, yp = (e / nx) * dy - y0;
-    const double scale = A * exp(-k * hypot(xp, yp));
-    uw[e] = -scale * vmax * (ca * xp + sa * yp);
-    vw[e] = -scale * vmax * (-sa * xp + ca * yp);
+    double u = 0.0, v = 0.0;
+    for (int i = 0; i < st.n; ++i) {
+        const double xp = (e % nx) * dx - st.x0[i], yp = (e / nx) * dy - st.y0[i];
+        const double scale = A * exp(-k * hypot(xp, yp));
+        u += -scale * st.vmax[i] * (ca * xp + sa * yp);
+        v += -scale * st.vmax[i] * (-sa * xp + ca * yp);
+    }
+    uw[e] = u;
+    vw[e] = v;
 }
 __global__ void k_zero_higher(double* f, int ne, int nc)
 {
@@ -3676,7 +3686,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         endRun();
     }
     // Subcycles in chunks until the momentum residual (which reads only h, A, u, v and the
-    // forcing) is at most rTarget or maxN have run.
+    // forcing) is at most rTarget or maxN have run. Chunks are at most `every` long; once
+    // the residual has fallen over a chunk, the next one ends where its current rate of
+    // decrease reaches rTarget (at least one subcycle), so the run stops within a few
+    // subcycles of the crossing instead of up to `every` past it.
     void subcyclesToResidual(double rTarget, int maxN, int every, int* done, double* rFinal) override
     {
         need();
@@ -3688,10 +3701,19 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         int n = 0;
         if (r > rTarget && maxN > 0) {
             beginRun();
+            double rPrev = r;
+            int last = 0;
             while (r > rTarget && n < maxN) {
-                const int step = std::min(every, maxN - n);
+                int step = std::min(every, maxN - n);
+                if (last > 0 && rPrev > r) {
+                    const double need = std::ceil((r - rTarget) / ((rPrev - r) / last));
+                    if (need < step)
+                        step = std::max(1, static_cast<int>(need));
+                }
+                rPrev = r;
                 runPasses(step);
                 n += step;
+                last = step;
                 residual(&r, nullptr, nullptr);
             }
             endRun();
@@ -4085,13 +4107,26 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     }
     void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) override
     {
+        stormWind(L, t, &xoff, &yoff, &vmax, 1);
+    }
+    // Every storm follows the reference cyclone's track (centre at half the box, moving by
+    // a fifth of it per day along the diagonal), shifted by its own offset.
+    void stormWind(double L, double t, const double* xoff, const double* yoff, const double* vmax, int n) override
+    {
         need();
+        require(n >= 1 && n <= kMaxStorms, "between 1 and " + std::to_string(kMaxStorms) + " storms");
+        require(xoff && yoff && vmax, "storm arrays are NULL");
         const double dx = L / mesh.nx, dy = L / mesh.ny, tf = t / 86400.0;
-        const double x0 = mesh.nx * dx * 0.5 * (1 + tf / 5.0) + xoff;
-        const double y0 = mesh.ny * dy * 0.5 * (1 + tf / 5.0) + yoff;
+        Storms st {};
+        st.n = n;
+        for (int i = 0; i < n; ++i) {
+            st.x0[i] = mesh.nx * dx * 0.5 * (1 + tf / 5.0) + xoff[i];
+            st.y0[i] = mesh.ny * dy * 0.5 * (1 + tf / 5.0) + yoff[i];
+            st.vmax[i] = vmax[i];
+        }
         {
             Timed tm(prof, s, "benchmark_wind");
-            k_benchmark_wind<<<blocksFor(ne), kBlock, 0, s>>>(mesh.nx, mesh.ny, dx, dy, x0, y0, vmax, dg0a.p, dg0b.p);
+            k_benchmark_wind<<<blocksFor(ne), kBlock, 0, s>>>(mesh.nx, mesh.ny, dx, dy, st, dg0a.p, dg0b.p);
             checkLaunch("benchmark_wind");
         }
         launchDG2CG<1>(dg0a.p, ua.p, 0, dg0b.p, va.p, 0);

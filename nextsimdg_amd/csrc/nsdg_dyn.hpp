@@ -28,6 +28,7 @@ struct DynBase {
     virtual void setPrecision(int p) = 0;
     virtual void setPairing(int mode) = 0;
     virtual void benchmarkWind(double L, double t, double xoff, double yoff, double vmax) = 0;
+    virtual void stormWind(double L, double t, const double* xoff, const double* yoff, const double* vmax, int n) = 0;
     virtual void resetDG0(const std::string& name) = 0;
     virtual void mass(double* vol, double* area) = 0;
     virtual void massNoAlloc(double* vol, double* area) = 0;

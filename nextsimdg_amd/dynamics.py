@@ -265,6 +265,12 @@ class MEVPDynamicsKernel:
     def benchmark_wind(self, L, t, xoff=0.0, yoff=0.0, vmax=30.0):
         check(self.lib.nsdg_dyn_benchmark_wind(self.h, float(L), float(t), float(xoff), float(yoff), float(vmax)))
 
+    def storm_wind(self, L, t, storms):
+        """The sum of benchmark cyclones, storms = [(xoff, yoff, vmax), ...] (at most 8)."""
+        a = np.ascontiguousarray(np.asarray(storms, dtype=np.float64).reshape(-1, 3).T)
+        check(self.lib.nsdg_dyn_storm_wind(self.h, float(L), float(t), dptr(a[0]), dptr(a[1]), dptr(a[2]),
+                                           a.shape[1]))
+
     def reset_dg0(self, name):
         check(self.lib.nsdg_dyn_reset_dg0(self.h, name.encode()))
 

@@ -76,6 +76,16 @@ def cyclone_offset(seed, L=L_BOX, frac=0.05):
     return tuple(rng.uniform(-frac * L, frac * L, 2))
 
 
+def storms(seed, n=3, L=L_BOX, peak=(10.0, 25.0), frac=0.3):
+    """n seeded storms, each a copy of the benchmark cyclone on its track shifted by up to
+    frac L, with a peak wind speed uniform in peak (m/s): rows (xoff, yoff, vmax), where
+    vmax = peak e (the cyclone's speed A vmax r e^(-k r) peaks at vmax / e, r = 1 / k)."""
+    rng = np.random.default_rng(seed + 54321)
+    off = rng.uniform(-frac * L, frac * L, (n, 2))
+    vmax = rng.uniform(peak[0], peak[1], n) * np.e
+    return np.column_stack([off, vmax])
+
+
 def benchmark_setup(nx, ny, seed=0, kind="plain"):
     """Everything a run needs: coords, mask, h, A, ocean u, v (all float64)."""
     coords, mask = box_coords(nx, ny), box_mask(nx, ny)

@@ -4,4 +4,4 @@ Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
 import this package, and only as the checker. The product (``nextsimdg_amd``)
 never imports, links or runs anything here.
 """
-from .oracle import MEB_DEFAULTS, MEB_ORDER, Oracle, OracleDyn, OracleMesh, OracleTransport, PARAM_ORDER, load  # noqa: F401
+from .oracle import MEB_DEFAULTS, MEB_ORDER, Oracle, OracleDyn, OracleMesh, OracleTransport, PARAM_ORDER, load, set_threads  # noqa: F401

@@ -5,6 +5,9 @@
 // advection tests (dynamics/test/Advection_test.cpp, AdvectionPeriodicBC_test.cpp)
 // and the benchmark forcing formulas (physics/src/modules/.../BenchmarkAtmosphere.cpp,
 // BenchmarkOcean.cpp) used as inputs.
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "ora_core.hpp"
 
 #include <cstdint>
@@ -596,6 +599,18 @@ int ora_dyn_update(void* d, double dt) { return guard([&] { static_cast<DynBase*
 int ora_dyn_advection(void* d, double dt) { return guard([&] { static_cast<DynBase*>(d)->advection(dt); }); }
 int ora_dyn_prepare_iteration(void* d) { return guard([&] { static_cast<DynBase*>(d)->prepareIteration(); }); }
 int ora_dyn_begin_iteration(void* d, double dt) { return guard([&] { static_cast<DynBase*>(d)->beginIteration(dt); }); }
+// OpenMP threads of the oracle's parallel loops (for timing on a stated thread count)
+int ora_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0)
+        omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 // Tools::MeanValue of hice and cice (ice volume and area)
 int ora_dyn_mass(void* d, double* out) { return guard([&] { static_cast<DynBase*>(d)->mass(out); }); }
 // Advection mass budget since the last reset: outflow h, outflow A, limiter h, limiter A

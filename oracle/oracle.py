@@ -83,6 +83,7 @@ def load():
         "ora_dyn_begin_iteration": (ctypes.c_int, [_vp, ctypes.c_double]),
         "ora_dyn_pass": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "ora_dyn_mass": (ctypes.c_int, [_vp, _dp]),
+        "ora_set_threads": (ctypes.c_int, [ctypes.c_int]),
         "ora_dyn_budget": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
         "ora_benchmark_wind": (None, [_sz, _sz, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, _dp, _dp]),
@@ -331,6 +332,11 @@ class OracleDyn:
         out = np.zeros(4)
         _check(self.lib.ora_dyn_budget(self.h, _p(out), int(bool(reset))))
         return dict(outflow_h=out[0], outflow_A=out[1], limiter_h=out[2], limiter_A=out[3])
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle (n <= 0 leaves it); returns the count in use."""
+    return load().ora_set_threads(int(n))
 
 
 class Oracle:

@@ -56,17 +56,21 @@ int main(int argc, char** argv)
             if (dyn->usesDamage())
                 store[nsdg::DAMAGE_IN] = store[nsdg::DAMAGE];
         }
-        auto sum = [](const nsdg::Field& f) {
-            double s = 0;
+        // every cell value of every output field, one field per line
+        auto print = [](const char* name, const nsdg::Field& f) {
+            std::printf("%s", name);
             for (double v : f)
-                s += v;
-            return s;
+                std::printf(" %.17g", v);
+            std::printf("\n");
         };
-        std::printf("%.17g %.17g %.17g %.17g %.17g %.17g", sum(store[nsdg::H_ICE]), sum(store[nsdg::C_ICE]),
-            sum(dyn->uice), sum(dyn->vice), sum(dyn->taux), sum(dyn->tauy));
+        print("hice", store[nsdg::H_ICE]);
+        print("cice", store[nsdg::C_ICE]);
+        print("u", dyn->uice);
+        print("v", dyn->vice);
+        print("uiostress", dyn->taux);
+        print("viostress", dyn->tauy);
         if (brittle)
-            std::printf(" %.17g", sum(dyn->damage));
-        std::printf("\n");
+            print("damage", dyn->damage);
     } catch (const nsdg::Error& e) {
         std::fprintf(stderr, "nsdg error %d: %s\n", e.code, e.what());
         return 1;

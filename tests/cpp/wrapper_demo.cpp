@@ -1,6 +1,6 @@
 // C++ caller of the gfx950 library through include/nsdg/nsdg_dyn.hpp: one module-level
-// step sequence of a small benchmark box; prints the sums of the cell means of u, v, hice,
-// cice (and damage with the brittle rheology). Usage: wrapper_demo [steps] [mevp|bbm]
+// step sequence of a small benchmark box; prints the cell means of u, v, hice, cice (and
+// damage with the brittle rheology), one field per line. Usage: wrapper_demo [steps] [mevp|bbm]
 #include "nsdg/nsdg_dyn.hpp"
 
 #include <cmath>
@@ -50,22 +50,19 @@ int main(int argc, char** argv)
             h = k.getDG0Data("hice");
             a = k.getDG0Data("cice");
         }
-        const auto u = k.getDG0Data("u"), v = k.getDG0Data("v");
-        double su = 0, sv = 0, sh = 0, sa = 0;
-        for (size_t e = 0; e < nx * ny; ++e) {
-            su += u[e];
-            sv += v[e];
-            sh += h[e];
-            sa += a[e];
-        }
-        std::printf("%.17g %.17g %.17g %.17g", su, sv, sh, sa);
-        if (brittle) {
-            double sd = 0;
-            for (double x : k.getDG0Data("damage"))
-                sd += x;
-            std::printf(" %.17g", sd);
-        }
-        std::printf("\n");
+        // every cell mean of every output field, one field per line
+        auto print = [](const char* name, const std::vector<double>& f) {
+            std::printf("%s", name);
+            for (double x : f)
+                std::printf(" %.17g", x);
+            std::printf("\n");
+        };
+        print("u", k.getDG0Data("u"));
+        print("v", k.getDG0Data("v"));
+        print("hice", h);
+        print("cice", a);
+        if (brittle)
+            print("damage", k.getDG0Data("damage"));
     } catch (const nsdg::Error& e) {
         std::fprintf(stderr, "nsdg error %d: %s\n", e.code, e.what());
         return 1;

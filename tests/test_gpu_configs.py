@@ -12,6 +12,12 @@ ulp it departs by ~1e-8 after one step and ~1e-4 after ten. Hence two checks:
     same inputs, every field (u, v, h, A, sigma) lies within 1e-10 or within 10x the
     spread of the oracle under that one-ulp perturbation, whichever is larger.
 
+Config 5 (2048 x 2048, three seeded storms of 10-25 m/s -- copies of the reference cyclone,
+BenchmarkAtmosphere.cpp:29-76 -- heterogeneous ice with open-water leads, FP32 bulk): the
+storm forcing equals the oracle's cyclones summed; five steps stay finite with exact
+Dirichlet zeros, repeat bit for bit, close the mass budget to 1e-13, and keep FP32 within
+3x the FP64 path's own spread.
+
 Config 3 (1024 x 1024, heterogeneous ice, FP32 variant): the same amplification holds, so
   * from one shared FP64 state, 1 and 10 FP32 subcycles stay within 1e-5 relative L2 of
     FP64 per field: the FP32 arithmetic's own error;
@@ -228,3 +234,77 @@ def test_config3_fp32_against_the_oracle_at_24():
     for n, e in errs.items():
         assert e < 1e-5, (n, e)
     assert max(errs.values()) > 1e-12
+
+
+def test_storm_wind_is_the_sum_of_reference_cyclones():
+    """nsdg_dyn_storm_wind against the oracle's cyclone (BenchmarkAtmosphere.cpp:29-76)
+    summed over the storms, on the CG nodes after the DG0 -> CG transfer."""
+    import nextsimdg_amd as nd
+    from nextsimdg_amd import inputs
+
+    _say("storm wind against summed oracle cyclones")
+    nx = ny = 40
+    s = inputs.benchmark_setup(nx, ny, 1)
+    st = inputs.storms(9)
+    g = _gpu(nx, s)
+    o = _oracle(nx, s)
+    for t in (0.0, 3 * 3600.0, 2.5 * 86400.0):
+        uw = sum(ORA.benchmark_wind(nx, ny, inputs.L_BOX, t, x, y, v)[0] for x, y, v in st)
+        vw = sum(ORA.benchmark_wind(nx, ny, inputs.L_BOX, t, x, y, v)[1] for x, y, v in st)
+        o.set_field("uwind", uw, 1)
+        o.set_field("vwind", vw, 1)
+        g.storm_wind(inputs.L_BOX, t, st)
+        for n in ("uwind", "vwind"):
+            assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-14, (n, t)
+    # peak speeds in the configured 10-25 m/s range (vmax / e at 100 km from a centre)
+    assert np.all((st[:, 2] / np.e >= 10.0) & (st[:, 2] / np.e <= 25.0))
+
+
+def test_config5_storms_and_leads_at_2048():
+    """Config 5's workload at its size: 2048 x 2048, three seeded storms, heterogeneous ice
+    with open-water leads, FP32 bulk. Five steps: finite, Dirichlet nodes exactly zero,
+    bit-for-bit repeatable, the advection mass budget closed to 1e-13 in every step, and
+    the FP32 state within 3x the FP64 path's own spread under a 1e-7 perturbation of h
+    (or 1e-5)."""
+    import bench
+    from nextsimdg_amd import inputs
+
+    nx, steps = 2048, 5
+    _say(f"config 5: {nx}^2, 3 storms, leads, FP32, {steps} steps")
+    st = inputs.storms(5)
+    runs = {}
+    for tag, prec, pert in (("fp32", 1, 0.0), ("fp32b", 1, 0.0), ("fp64", 0, 0.0), ("floor", 0, 1e-7)):
+        k, _, L = bench.make_kernel(nx, nx, 5, prec, "leads", pert)
+        if tag == "fp32":
+            k.set_budget(True)
+        for i in range(steps):
+            k.reset_dg0("hice")
+            k.reset_dg0("cice")
+            k.storm_wind(L, 120.0 * i, st)
+            if tag == "fp32":
+                h0, a0 = k.mass()
+            k.update(120.0)
+            if tag == "fp32":
+                h1, a1 = k.mass()
+                b = k.budget(reset=True)
+                eh = abs((h1 - h0) - (b["limiter_h"] - b["outflow_h"])) / h0
+                ea = abs((a1 - a0) - (b["limiter_A"] - b["outflow_A"])) / a0
+                _say(f"  step {i}: budget closure h {eh:.1e} A {ea:.1e}")
+                assert eh <= 1e-13 and ea <= 1e-13, (i, eh, ea)
+        runs[tag] = _fields_gpu(k)
+        if tag == "fp32":
+            u = runs[tag]["u"].reshape(2 * nx + 1, 2 * nx + 1)
+            assert np.all(np.isfinite(u)) and np.abs(u).max() > 1e-3
+            # the CG nodes on the edges between the land ring and the ocean are Dirichlet nodes
+            lo, hi = 2, 2 * nx - 2
+            assert np.all(u[lo, lo:hi + 1] == 0) and np.all(u[hi, lo:hi + 1] == 0)
+            assert np.all(u[lo:hi + 1, lo] == 0) and np.all(u[lo:hi + 1, hi] == 0)
+        del k
+    for n in runs["fp32"]:
+        assert np.array_equal(runs["fp32"][n], runs["fp32b"][n]), n
+    e32 = {n: rel_l2(runs["fp32"][n], runs["fp64"][n]) for n in runs["fp64"]}
+    efl = {n: rel_l2(runs["floor"][n], runs["fp64"][n]) for n in runs["fp64"]}
+    _say("  fp32:  " + " ".join(f"{n}={e:.1e}" for n, e in e32.items()))
+    _say("  floor: " + " ".join(f"{n}={e:.1e}" for n, e in efl.items()))
+    for n in e32:
+        assert e32[n] <= max(1e-5, 3.0 * efl[n]), (n, e32[n], efl[n])
