@@ -323,6 +323,18 @@ template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
     out[n] = acc;
 }
 
+// Node tiles: a block owns the CG nodes of the lower-left 2x2 (CG2) node blocks of TX x TY
+// elements (the last tile row/column also owns the domain edge) and gathers the weak
+// divergence of the tile and a one-element left/bottom halo (HX x HY elements) from LDS.
+template <int CG, int TX_, int TY_> struct TileShape {
+    static constexpr int TX = TX_, TY = TY_;
+    static constexpr int HX = TX + 1, HY = TY + 1, N = cgdofs(CG);
+    static constexpr int CW = CG * TX;                        // owned node columns
+    static constexpr int TPR = CW < kBlock ? CW : kBlock;     // threads per node row
+    static constexpr int RS = kBlock / TPR;                   // node rows per block sweep
+    static constexpr int NB = (CG * TY + RS - 1) / RS;        // nodes per thread per sweep
+};
+
 // ---------------------------------------------------------------- DG -> CG
 // Interpolations::DG2CG (Interpolations.cpp:129-204): Lagrange-point values weighted
 // 1/4 (corner), 1/2 (edge midpoint), 1 (centre), gathered odd element rows first,
@@ -368,6 +380,84 @@ __global__ __launch_bounds__(kBlock) void k_dg2cg(int nx, int ny, int ne, const 
     if (s2)
         d2[n] = clampv(a2 * f, clamp2);
 }
+// The same, tiled (31 x 7 elements per block, as the one-pass subcycle): every element of
+// the tile and of its one-element left/bottom halo evaluates its weighted Lagrange-point
+// values into LDS once, one element per thread, then the block's nodes (those of its
+// elements' lower-left blocks, plus the domain edge for the last tile row and column) weight
+// and sum them in the order of k_dg2cg, with the same operations; each element's
+// coefficients are read once (plus the halo) instead of by each of its nodes.
+template <int CG, int D>
+__global__ __launch_bounds__(kBlock) void k_dg2cg_tiled(int nx, int ny, int ne, const double* s1, double* d1,
+    int clamp1, const double* s2, double* d2, int clamp2)
+{
+    using T = TileShape<CG, 31, 7>;
+    constexpr int N = T::N;
+    __shared__ double val[T::HX * T::HY][N][2];
+    const int tilesX = (nx + T::TX - 1) / T::TX, tilesY = (ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    if (tile < 0)
+        return;
+    const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+    {
+        const int le = threadIdx.x;
+        const int ex = ex0 - 1 + le % T::HX, ey = ey0 - 1 + le / T::HX;
+        if (ex >= 0 && ex < nx && ey >= 0 && ey < ny) {
+            const int e = ex + nx * ey;
+            double c1[D], c2[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                c1[i] = s1[static_cast<size_t>(i) * ne + e];
+                c2[i] = s2 ? s2[static_cast<size_t>(i) * ne + e] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                double t1 = 0, t2 = 0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    const double l = PsiL<D, CG + 1>::v[i][j];
+                    t1 += c1[i] * l;
+                    t2 += c2[i] * l;
+                }
+                val[le][j][0] = t1;
+                val[le][j][1] = t2;
+            }
+        }
+    }
+    __syncthreads();
+    const int row = CG * nx + 1;
+    const int nx0 = CG * ex0, ny0 = CG * ey0;
+    const int W = (ex0 + T::TX >= nx) ? row - nx0 : CG * T::TX;
+    const int H = (ey0 + T::TY >= ny) ? CG * ny + 1 - ny0 : CG * T::TY;
+    auto clampv = [](double x, int c) {
+        if (c == 1)
+            return fmax(x, 1.e-4);
+        if (c == 2)
+            return fmax(fmin(x, 1.0), 1.e-4);
+        return x;
+    };
+    for (int idx = threadIdx.x; idx < W * H; idx += kBlock) {
+        const int nix = nx0 + idx % W, niy = ny0 + idx / W;
+        double a1 = 0, a2 = 0;
+        forEachAdjacentXY<CG, false>(nix, niy, nx, ny, [&](int ex, int ey, int j) {
+            const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
+            const int jx = j % (CG + 1), jy = j / (CG + 1);
+            const double w = CG == 1 ? 0.25 : ((jx == 1 && jy == 1) ? 1.0 : ((jx == 1 || jy == 1) ? 0.5 : 0.25));
+            a1 += w * val[le][j][0];
+            a2 += w * val[le][j][1];
+        });
+        double f = 1.0;
+        if (niy == 0 || niy == CG * ny)
+            f *= 2.0;
+        if (nix == 0 || nix == CG * nx)
+            f *= 2.0;
+        const int n = nix + row * niy;
+        d1[n] = clampv(a1 * f, clamp1);
+        if (s2)
+            d2[n] = clampv(a2 * f, clamp2);
+    }
+}
+static_assert(TileShape<2, 31, 7>::HX * TileShape<2, 31, 7>::HY == kBlock, "one element per thread");
+
 template <int CG> __global__ void k_clamp_HA(double* H, double* A, int nn)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1290,17 +1380,6 @@ __device__ __forceinline__ void momentumFrozen(const NodeArgs& a, const NodeFz& 
     vn = inv * (c1 * a.beta * in.v + in.qf.y + drag * in.ocn.y + cor * in.v + dSy);
 }
 
-// Node tiles: a block owns the CG nodes of the lower-left 2x2 (CG2) node blocks of TX x TY
-// elements (the last tile row/column also owns the domain edge) and gathers the weak
-// divergence of the tile and a one-element left/bottom halo (HX x HY elements) from LDS.
-template <int CG, int TX_, int TY_> struct TileShape {
-    static constexpr int TX = TX_, TY = TY_;
-    static constexpr int HX = TX + 1, HY = TY + 1, N = cgdofs(CG);
-    static constexpr int CW = CG * TX;                        // owned node columns
-    static constexpr int TPR = CW < kBlock ? CW : kBlock;     // threads per node row
-    static constexpr int RS = kBlock / TPR;                   // node rows per block sweep
-    static constexpr int NB = (CG * TY + RS - 1) / RS;        // nodes per thread per sweep
-};
 // Fused subcycle: 31 x 7 elements, so that tile and halo are exactly one element per thread.
 template <int CG> using FusedTile = TileShape<CG, 31, 7>;
 static_assert(FusedTile<2>::HX * FusedTile<2>::HY == kBlock, "fused tile must give one element per thread");
@@ -1441,6 +1520,110 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
     __syncthreads();
     nodeTileUpdate<CG, T>(
         a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
+}
+
+// R_x^2 + R_y^2 and (M_L A tau_a)^2 of node n, added to r2 and d2.
+__device__ __forceinline__ void residualNode(const NodeArgs& a, int n, double dSx, double dSy, double& r2, double& d2)
+{
+    const double u = a.u[n], v = a.v[n], uo = a.uo[n], vo = a.vo[n], ua = a.ua[n], va = a.va[n];
+    const double H = a.H[n], A = a.A[n], ML = a.ML[n];
+    const double absatm = sqrt(ua * ua + va * va);
+    const double du = uo - u, dv = v - vo;
+    const double absocn = sqrt(du * du + dv * dv);
+    const double c1 = a.rho_ice * H / a.dt, rhoH = a.rho_ice * H;
+    const double tax = a.F_atm * absatm * ua, tay = a.F_atm * absatm * va;
+    const double rx = ML
+            * (c1 * (u - a.u0[n]) - A * (tax + a.F_ocean * absocn * (uo - u)) + rhoH * a.fc * u
+                + rhoH * a.gravity * a.gx[n])
+        - dSx;
+    const double ry = ML
+            * (c1 * (v - a.v0[n]) - A * (tay + a.F_ocean * absocn * (vo - v)) - rhoH * a.fc * v
+                + rhoH * a.gravity * a.gy[n])
+        - dSy;
+    const double fx = ML * A * tax, fy = ML * A * tay;
+    r2 += rx * rx + ry * ry;
+    d2 += fx * fx + fy * fy;
+}
+// The momentum residual in one pass over a uniform mesh without periodic segments: the
+// element phase of k_mevp_fused with alpha = 1 and no previous stress (Pi sigma_VP(eps(u)),
+// as k_vp_stress), its weak divergence into LDS, then every owned non-Dirichlet node forms
+// R from the LDS contributions (in nodeTileUpdate's order) and adds R^2 and
+// (M_L A tau_a)^2 to its thread's compensated sums; one (sum, compensation) pair per field
+// and block for k_sum_final<2>. Replaces k_vp_stress, k_divergence and k_residual_nodes,
+// and the stress and divergence never touch HBM.
+template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_residual_fused(FusedArgs f, double* part)
+{
+    using T = FusedTile<CG>;
+    constexpr int N = T::N, S = cg2dgstress(CG);
+    __shared__ double contrib[T::HX * T::HY][N][2];
+    __shared__ double red[4][kBlock];
+    const NodeArgs& a = f.n;
+    const int tilesX = (a.nx + T::TX - 1) / T::TX, tilesY = (a.ny + T::TY - 1) / T::TY;
+    const int tile = xcdTile(blockIdx.x, tilesX * tilesY);
+    double sum[2] = { 0.0, 0.0 }, comp[2] = { 0.0, 0.0 };
+    if (tile >= 0) {
+        const int ex0 = (tile % tilesX) * T::TX, ey0 = (tile / tilesX) * T::TY;
+        {
+            const int le = threadIdx.x;
+            const int ex = ex0 - 1 + le % T::HX, ey = ey0 - 1 + le / T::HX;
+            double tx[N], ty[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                tx[j] = ty[j] = 0.0;
+            if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
+                const int e = ex + a.nx * ey;
+                if (a.ocean[e]) {
+                    Acc<double> acc1[S], acc2[S], acc3[S];
+                    elementAffineCompute<CG, DG, double, double, true>(f.e, e, acc1, acc2, acc3);
+                    double s11[S], s12[S], s22[S];
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        s11[k] = acc1[k].get();
+                        s12[k] = acc2[k].get();
+                        s22[k] = acc3[k].get();
+                    }
+                    divergenceAffineFrom<CG, double>(a, s11, s12, s22, tx, ty);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                contrib[le][j][0] = tx[j];
+                contrib[le][j][1] = ty[j];
+            }
+        }
+        __syncthreads();
+        const int row = CG * a.nx + 1;
+        const int nx0 = CG * ex0, ny0 = CG * ey0;
+        const int W = (ex0 + T::TX >= a.nx) ? row - nx0 : CG * T::TX;
+        const int H = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 - ny0 : CG * T::TY;
+        for (int idx = threadIdx.x; idx < W * H; idx += kBlock) {
+            const int nix = nx0 + idx % W, niy = ny0 + idx / W;
+            const int n = nix + row * niy;
+            if (a.dirNode[n])
+                continue;
+            double dx = 0.0, dy = 0.0;
+            forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
+                const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
+                dx -= contrib[le][j][0];
+                dy -= contrib[le][j][1];
+            });
+            double r2 = 0.0, d2 = 0.0;
+            residualNode(a, n, dx, dy, r2, d2);
+            double t, err;
+            twoSum(sum[0], r2, t, err);
+            sum[0] = t;
+            comp[0] += err;
+            twoSum(sum[1], d2, t, err);
+            sum[1] = t;
+            comp[1] += err;
+        }
+    }
+    blockTwoSum<2>(red, sum, comp);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 2; ++k) {
+            part[(blockIdx.x * 2 + k) * 2] = sum[k];
+            part[(blockIdx.x * 2 + k) * 2 + 1] = comp[k];
+        }
 }
 
 // Two mEVP subcycles in one pass over a uniform mesh (FP64), the state read from and
@@ -2038,24 +2221,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_nodes(NodeArgs a, const dou
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.nn; n += gridDim.x * blockDim.x) {
         if (a.dirNode[n])
             continue;
-        const double u = a.u[n], v = a.v[n], uo = a.uo[n], vo = a.vo[n], ua = a.ua[n], va = a.va[n];
-        const double H = a.H[n], A = a.A[n], ML = a.ML[n];
-        const double absatm = sqrt(ua * ua + va * va);
-        const double du = uo - u, dv = v - vo;
-        const double absocn = sqrt(du * du + dv * dv);
-        const double c1 = a.rho_ice * H / a.dt, rhoH = a.rho_ice * H;
-        const double tax = a.F_atm * absatm * ua, tay = a.F_atm * absatm * va;
-        const double rx = ML
-                * (c1 * (u - a.u0[n]) - A * (tax + a.F_ocean * absocn * (uo - u)) + rhoH * a.fc * u
-                    + rhoH * a.gravity * a.gx[n])
-            - dSx[n];
-        const double ry = ML
-                * (c1 * (v - a.v0[n]) - A * (tay + a.F_ocean * absocn * (vo - v)) - rhoH * a.fc * v
-                    + rhoH * a.gravity * a.gy[n])
-            - dSy[n];
-        const double fx = ML * A * tax, fy = ML * A * tay;
-        r2 += rx * rx + ry * ry;
-        d2 += fx * fx + fy * fy;
+        residualNode(a, n, dSx[n], dSy[n], r2, d2);
     }
     r2 = waveSum(r2);
     d2 = waveSum(d2);
@@ -3117,7 +3283,11 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         uAlt.alloc(nn);
         vAlt.alloc(nn);
         sigAlt.alloc(3 * S * ne);
-        rpart.alloc(4 * kResidualBlocks);
+        {
+            using T = FusedTile<CG>;
+            const size_t tiles = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+            rpart.alloc(4 * std::max<size_t>(kResidualBlocks, tiles));
+        }
         rsum.alloc(2);
         for (auto* b : { &h, &A, &tmpDG, &tmpDG2 })
             deviceFill(b->p, b->n, 0.0, s);
@@ -3258,7 +3428,9 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     template <int D> void launchDG2CG(const double* s1, double* d1, int c1, const double* s2, double* d2, int c2)
     {
         Timed t(prof, s, "dg2cg");
-        k_dg2cg<CG, D><<<blocksFor(nn), kBlock, 0, s>>>(mesh.nx, mesh.ny, static_cast<int>(ne), s1, d1, c1, s2, d2, c2);
+        using T = TileShape<CG, 31, 7>;
+        const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+        k_dg2cg_tiled<CG, D><<<grid, kBlock, 0, s>>>(mesh.nx, mesh.ny, static_cast<int>(ne), s1, d1, c1, s2, d2, c2);
         checkLaunch("dg2cg");
     }
     void setField(const std::string& n, const double* data, int ncomp) override
@@ -3968,7 +4140,24 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         need();
         needMEVP("the momentum residual");
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
-        {
+        if (fusedSubcycle() && mesh.uniform) {
+            Timed t(prof, s, "residual");
+            k_strength<CG, DG, double><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar,
+                strength.p);
+            checkLaunch("strength");
+            FusedArgs f {};
+            f.e = elemArgs();
+            f.e.invAlpha = 1.0;
+            f.e.relax = 0.0;
+            f.n = nodeArgs();
+            using T = FusedTile<CG>;
+            const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+            require(rpart.n >= 4 * static_cast<size_t>(grid), "residual partials too small", NSDG_ERR_STATE);
+            k_residual_fused<CG, DG><<<grid, kBlock, 0, s>>>(f, rpart.p);
+            checkLaunch("residual_fused");
+            k_sum_final<2><<<1, kBlock, 0, s>>>(rpart.p, static_cast<int>(grid), rsum.p);
+            checkLaunch("residual_final");
+        } else {
             Timed t(prof, s, "residual");
             vpStressAsync(sigVP.p);
             NodeArgs a = nodeArgs();

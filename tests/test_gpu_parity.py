@@ -153,6 +153,24 @@ def _set_random_state(o, g, rng, S, dg):
 
 
 @pytest.mark.parametrize("cg,dg", CASES)
+@pytest.mark.parametrize("nx,ny", [(3, 2), (31, 7), (32, 8), (62, 14), (63, 15)])
+def test_dg2cg_tiles_match_oracle(cg, dg, nx, ny):
+    """DG2CG (Interpolations.cpp:129-204) runs in tiles of 31 x 7 elements: meshes smaller
+    than a tile, tile multiples and one past them. The forcing's DG0 -> CG transfer and the
+    clamped cgH, cgA of prepareIteration against the oracle."""
+    _say(f"dg2cg tiles cg={cg} dg={dg} {nx}x{ny}")
+    o, g = _pair(cg, dg, nx, ny, mask=_box_mask(nx, ny) if nx > 2 and ny > 2 else np.ones(nx * ny),
+                 coords=box_coords(nx, ny, 4000.0))
+    _set_random_state(o, g, np.random.default_rng(nx * 100 + ny), o.S, dg)
+    for n in ("uwind", "vwind", "uocean", "vocean"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-15, n
+    o.prepare_iteration()
+    g.prepareIteration()
+    for n in ("cgH", "cgA"):
+        assert rel_l2(g.get_cg(n), o.get_cg(n)) < 1e-14, n
+
+
+@pytest.mark.parametrize("cg,dg", CASES)
 @pytest.mark.parametrize("uniform", [True, False])
 def test_single_passes_match_oracle(cg, dg, uniform):
     _say(f"single passes cg={cg} dg={dg} uniform={uniform}")

@@ -62,12 +62,14 @@ def _dirichlet_nodes(cg, dg, nx, ny, coords, mask):
 
 
 @pytest.mark.parametrize("cg,dg", [(2, 6), (1, 3)])
-@pytest.mark.parametrize("uniform", [True, False])
-def test_residual_matches_oracle(cg, dg, uniform):
+@pytest.mark.parametrize("uniform,nx,ny", [(True, 18, 14), (True, 63, 15), (False, 18, 14)])
+def test_residual_matches_oracle(cg, dg, uniform, nx, ny):
+    """Uniform meshes take the one-pass residual kernel (tiles of 31 x 7 elements: one
+    tile, and one past two tile multiples), general meshes the split passes."""
     import nextsimdg_amd as nd
 
-    _say(f"residual cg={cg} dg={dg} uniform={uniform}")
-    nx, ny, dt = 18, 14, 120.0
+    _say(f"residual cg={cg} dg={dg} uniform={uniform} {nx}x{ny}")
+    dt = 120.0
     coords = box_coords(nx, ny, 4000.0)
     if not uniform:
         coords[:, 0] += 150.0 * np.sin(coords[:, 1] / 9000.0)
