@@ -265,7 +265,7 @@ def fp32_variant(nx, steps, warmup, err_steps, seed=0):
 WEIGHTS = os.path.join(ROOT, "nextsimdg_amd", "data", "warm_start_256.npy")
 
 
-def learned_vs_plain(nx, kinds=("heterogeneous", "leads", "plain"), seed=5000, spinup=2, check_every=25):
+def learned_vs_plain(nx, kinds=("heterogeneous", "leads", "plain"), seed=5000, spinup=2, check_every=50):
     """Speed at equal momentum residual: plain FP64 mEVP (100 subcycles from the cold start)
     against the FP32 path started by the trained network with the residual gate, iterating
     until r <= r(plain). Both from the same saved state at the start of a timestep."""
@@ -475,7 +475,7 @@ def budget_leg(nx, steps, seed=0):
     return out
 
 
-def config5_leg(nx, seed=5, spinup=2, check_every=25):
+def config5_leg(nx, seed=5, spinup=2, check_every=50):
     """Config 5: 2048 x 2048 box, three seeded storms (10-25 m/s), heterogeneous ice with
     open-water leads. One timestep after `spinup` plain steps, from one saved state: plain
     FP64 mEVP (100 subcycles) against FP32 with the gated learned start iterating until the
