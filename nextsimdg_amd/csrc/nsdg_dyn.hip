@@ -3861,7 +3861,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // forcing) is at most rTarget or maxN have run. Chunks are at most `every` long; once
     // the residual has fallen over a chunk, the next one ends where its current rate of
     // decrease reaches rTarget (at least one subcycle), so the run stops within a few
-    // subcycles of the crossing instead of up to `every` past it.
+    // subcycles of the crossing instead of up to `every` past it; a residual within 1% of
+    // the target that did not fall over the last chunk gets chunks of at most 5.
     void subcyclesToResidual(double rTarget, int maxN, int every, int* done, double* rFinal) override
     {
         need();
@@ -3881,6 +3882,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
                     const double need = std::ceil((r - rTarget) / ((rPrev - r) / last));
                     if (need < step)
                         step = std::max(1, static_cast<int>(need));
+                } else if (last > 0 && r - rTarget <= 0.01 * rTarget) {
+                    step = std::min(step, 5); // within 1% of the target but not falling: short chunks
                 }
                 rPrev = r;
                 runPasses(step);
