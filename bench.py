@@ -250,8 +250,8 @@ def fp32_variant(nx, steps, warmup, err_steps, seed=0):
     iters = nx * nx * 100 * steps
     return {
         "workload": f"mEVP dynamics step, {nx}x{nx} CG2/DG2, heterogeneous ice, FP32 stress/strength storage "
-                    "and element arithmetic (Kahan-compensated integrals, stochastic rounding of the stored "
-                    "stress), FP64 velocity update",
+                    "and element arithmetic (compensated relaxed-stress sum, stochastic rounding of the "
+                    "stored stress), FP64 velocity update",
         "value": round(iters / t32 / 1e6, 3),
         "fp64_value": round(iters / t64 / 1e6, 3),
         "speedup_vs_fp64": round(t64 / t32, 3),

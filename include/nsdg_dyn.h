@@ -54,8 +54,9 @@ typedef struct nsdg_params {
 
 /* Arithmetic of the element (rheology) pass. FP64 everywhere is the reference path.
  * NSDG_PREC_FP32 runs strain rates, Delta/viscosities, stress and the basis
- * contractions in FP32 with compensated element sums, keeping the velocity update,
- * lumped-mass inversion and conservation sums in FP64. */
+ * contractions in FP32 (the relaxed stress sum compensated, the stored stress rounded
+ * stochastically), keeping the velocity update, lumped-mass inversion and conservation
+ * sums in FP64. */
 #define NSDG_PREC_FP64 0
 #define NSDG_PREC_FP32 1
 

@@ -832,8 +832,8 @@ __device__ __forceinline__ constexpr double invMassRef(int s)
 {
     return s == 0 ? 1.0 : (s <= 2 ? 12.0 : (s <= 4 ? 180.0 : (s == 5 ? 144.0 : 2160.0)));
 }
-// Element integrals are accumulated in R: plainly in FP64, with Kahan compensation in FP32
-// (the exact sum is s - c; initProduct keeps the rounding error of a product as well).
+// The relaxed stress sum: plain in FP64; in FP32 compensated (the exact sum is s - c;
+// initProduct and addProduct keep the rounding error of a product as well).
 template <class R> struct Acc {
     R s;
     __device__ __forceinline__ void init(R v) { s = v; }
@@ -878,9 +878,11 @@ template <> struct Acc<float> {
 // the same error every subcycle). The FP32 stress is therefore stored with unbiased
 // stochastic rounding: x is rounded away from zero with probability (|x| - |rz(x)|) / ulp,
 // the random bits a hash of (element, coefficient, subcycle), so runs are reproducible.
-__device__ __forceinline__ uint32_t mixBits(uint32_t e, uint32_t k, uint32_t it)
+// One full hash per element and subcycle; the coefficient's bits from it by a Weyl step
+// and one multiply-xorshift round.
+__device__ __forceinline__ uint32_t elementSeed(uint32_t e, uint32_t it)
 {
-    uint32_t h = e * 0x9E3779B1u ^ (k + 1u) * 0x85EBCA77u ^ (it + 1u) * 0xC2B2AE3Du;
+    uint32_t h = e * 0x9E3779B1u ^ (it + 1u) * 0xC2B2AE3Du;
     h ^= h >> 16;
     h *= 0x7feb352du;
     h ^= h >> 15;
@@ -888,20 +890,34 @@ __device__ __forceinline__ uint32_t mixBits(uint32_t e, uint32_t k, uint32_t it)
     h ^= h >> 16;
     return h;
 }
+__device__ __forceinline__ uint32_t coefficientBits(uint32_t seed, uint32_t k)
+{
+    uint32_t x = seed + (k + 1u) * 0x9E3779B9u;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    return x;
+}
+// For |x| in the normal FP32 range: the FP32 pattern of x truncated is bits 29-63 of its
+// FP64 magnitude with the exponent rebiased (1023 -> 127); adding a uniform 29-bit integer
+// below that cut first carries into the kept bits with probability (low 29 bits) / 2^29,
+// which is (|x| - |rz(x)|) / ulp exactly. Outside that range, round to nearest.
 __device__ __forceinline__ float roundStochastic(double x, uint32_t bits)
 {
-    const float f = __double2float_rz(x);
-    const double r = fabs(x - static_cast<double>(f));
-    const float g = nextafterf(f, x > 0.0 ? INFINITY : -INFINITY);
-    const double ulp = fabs(static_cast<double>(g) - static_cast<double>(f));
-    return static_cast<double>(bits >> 8) * 0x1p-24 * ulp < r ? g : f;
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(x));
+    const uint64_t m = (b & 0x7fffffffffffffffull) + (bits & 0x1fffffffu);
+    const uint32_t sign = static_cast<uint32_t>(b >> 32) & 0x80000000u;
+    const uint32_t f = (static_cast<uint32_t>(m >> 29) - (896u << 23)) | sign;
+    const double ax = fabs(x);
+    return ax >= 0x1p-126 && ax < 0x1p128 ? __uint_as_float(f) : static_cast<float>(x);
 }
 template <class R, class C> __device__ __forceinline__ R storeValue(const Acc<C>& a, int e, int k, uint32_t it)
 {
     if constexpr (sizeof(R) == 8)
         return a.get();
     else
-        return roundStochastic(a.exact(), mixBits(static_cast<uint32_t>(e), static_cast<uint32_t>(k), it));
+        return roundStochastic(
+            a.exact(), coefficientBits(elementSeed(static_cast<uint32_t>(e), it), static_cast<uint32_t>(k)));
 }
 __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
 __device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
@@ -946,27 +962,16 @@ __device__ __forceinline__ void elementAffineCore(
     const C xiX = static_cast<C>(a.xiX), etaX = static_cast<C>(a.etaX);
     const C xiY = static_cast<C>(a.xiY), etaY = static_cast<C>(a.etaY);
     const C DM2 = static_cast<C>(a.DM2);
-    // The relaxed previous stress. FP64: the projected terms are summed first and
-    // (1 - 1/alpha) sigma is added at the end (the order of MEVPStressUpdateStep.hpp:78-116);
-    // the old stress is read only then, which keeps 3 S registers free through the strain.
-    // FP32: sigma - sigma/alpha with the same rounded 1/alpha that scales the new term, so
-    // that the fixed point is the projected sigma_VP exactly (a separately rounded
-    // 1 - 1/alpha would move it by up to 2^-24 alpha relative).
+    // The projected terms sum_q PSI_s(q) w_q / m_s r(q) of each coefficient, summed plainly:
+    // in acc (FP64), in t (FP32).
+    C t1[S], t2[S], t3[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if constexpr (VP || sizeof(C) == 8) {
+        t1[s] = t2[s] = t3[s] = 0;
+        if constexpr (sizeof(C) == 8) {
             acc1[s].init(0);
             acc2[s].init(0);
             acc3[s].init(0);
-        } else {
-            const C o1 = old(0, s), o2 = old(1, s), o3 = old(2, s);
-            const C mInvAlpha = -static_cast<C>(a.invAlpha);
-            acc1[s].init(o1);
-            acc2[s].init(o2);
-            acc3[s].init(o3);
-            acc1[s].addProduct(o1, mInvAlpha);
-            acc2[s].addProduct(o2, mInvAlpha);
-            acc3[s].addProduct(o3, mInvAlpha);
         }
     }
 #pragma unroll
@@ -1033,18 +1038,50 @@ __device__ __forceinline__ void elementAffineCore(
                 const double bs = bd[s] * invMassRef(s);
                 if (bs == 0.0) // basis functions vanishing at this Gauss point (a literal test)
                     continue;
-                acc1[s].add(static_cast<C>(bs) * r11);
-                acc2[s].add(static_cast<C>(bs) * r12);
-                acc3[s].add(static_cast<C>(bs) * r22);
+                if constexpr (sizeof(C) == 8) {
+                    acc1[s].add(static_cast<C>(bs) * r11);
+                    acc2[s].add(static_cast<C>(bs) * r12);
+                    acc3[s].add(static_cast<C>(bs) * r22);
+                } else {
+                    t1[s] += static_cast<C>(bs) * r11;
+                    t2[s] += static_cast<C>(bs) * r12;
+                    t3[s] += static_cast<C>(bs) * r22;
+                }
             }
         }
     }
-    if constexpr (sizeof(C) == 8 && !VP) {
+    // The relaxed previous stress, read only now (3 S registers stay free through the strain).
+    // FP64: (1 - 1/alpha) sigma + t, the order of MEVPStressUpdateStep.hpp:78-116. FP32:
+    // sigma - sigma/alpha + t in a compensated sum, with the same rounded 1/alpha that scales
+    // the new term, so that the fixed point is the projected sigma_VP exactly (a separately
+    // rounded 1 - 1/alpha would move it by up to 2^-24 alpha relative). The rounding inside t
+    // moves the fixed point by as much and is not amplified by alpha; a rounding of the
+    // relaxed sum itself would be, which the compensation (and the unbiased stochastic
+    // rounding of the stored result) prevents.
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            acc1[s].s = old(0, s) * a.relax + acc1[s].s;
-            acc2[s].s = old(1, s) * a.relax + acc2[s].s;
-            acc3[s].s = old(2, s) * a.relax + acc3[s].s;
+    for (int s = 0; s < S; ++s) {
+        if constexpr (sizeof(C) == 8) {
+            if constexpr (!VP) {
+                acc1[s].s = old(0, s) * a.relax + acc1[s].s;
+                acc2[s].s = old(1, s) * a.relax + acc2[s].s;
+                acc3[s].s = old(2, s) * a.relax + acc3[s].s;
+            }
+        } else if constexpr (VP) {
+            acc1[s].init(t1[s]);
+            acc2[s].init(t2[s]);
+            acc3[s].init(t3[s]);
+        } else {
+            const C o1 = old(0, s), o2 = old(1, s), o3 = old(2, s);
+            const C mInvAlpha = -static_cast<C>(a.invAlpha);
+            acc1[s].init(o1);
+            acc2[s].init(o2);
+            acc3[s].init(o3);
+            acc1[s].addProduct(o1, mInvAlpha);
+            acc2[s].addProduct(o2, mInvAlpha);
+            acc3[s].addProduct(o3, mInvAlpha);
+            acc1[s].add(t1[s]);
+            acc2[s].add(t2[s]);
+            acc3[s].add(t3[s]);
         }
     }
 }
@@ -1275,22 +1312,16 @@ __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R*
     constexpr int S = cg2dgstress(CG), NG = gp1d(S), N = cgdofs(CG), K = CG + 1;
     const R xiX = static_cast<R>(a.xiX), etaX = static_cast<R>(a.etaX);
     const R xiY = static_cast<R>(a.xiY), etaY = static_cast<R>(a.etaY);
-    Acc<R> sx[N], sy[N];
+    R sx[N], sy[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        sx[j].init(0);
-        sy[j].init(0);
-    }
+    for (int j = 0; j < N; ++j)
+        sx[j] = sy[j] = 0;
 #pragma unroll
     for (int gy = 0; gy < NG; ++gy) {
-        Acc<R> ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
+        R ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
 #pragma unroll
-        for (int jx = 0; jx < K; ++jx) {
-            ax[jx].init(0);
-            bx[jx].init(0);
-            ay[jx].init(0);
-            by[jx].init(0);
-        }
+        for (int jx = 0; jx < K; ++jx)
+            ax[jx] = bx[jx] = ay[jx] = by[jx] = 0;
 #pragma unroll
         for (int gx = 0; gx < NG; ++gx) {
             double b[8];
@@ -1310,10 +1341,10 @@ __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R*
             for (int jx = 0; jx < K; ++jx) {
                 const R ph = static_cast<R>(cg_1d(CG, jx, gauss_point(NG, gx)));
                 const R dp = static_cast<R>(cg_1d_dx(CG, jx, gauss_point(NG, gx)));
-                ax[jx].add(dp * fxXi);  // d/dxi part, integrated along x
-                bx[jx].add(ph * fxEta); // d/deta part
-                ay[jx].add(dp * fyXi);
-                by[jx].add(ph * fyEta);
+                ax[jx] += dp * fxXi;  // d/dxi part, integrated along x
+                bx[jx] += ph * fxEta; // d/deta part
+                ay[jx] += dp * fyXi;
+                by[jx] += ph * fyEta;
             }
         }
 #pragma unroll
@@ -1322,15 +1353,15 @@ __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R*
             const R dp = static_cast<R>(cg_1d_dx(CG, jy, gauss_point(NG, gy)));
 #pragma unroll
             for (int jx = 0; jx < K; ++jx) {
-                sx[jx + K * jy].add(ph * ax[jx].get() + dp * bx[jx].get());
-                sy[jx + K * jy].add(ph * ay[jx].get() + dp * by[jx].get());
+                sx[jx + K * jy] += ph * ax[jx] + dp * bx[jx];
+                sy[jx + K * jy] += ph * ay[jx] + dp * by[jx];
             }
         }
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        tx[j] = sx[j].get();
-        ty[j] = sy[j].get();
+        tx[j] = sx[j];
+        ty[j] = sy[j];
     }
 }
 

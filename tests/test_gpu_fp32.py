@@ -1,9 +1,10 @@
 """The FP32 variant against the FP64 path, both on the GPU, on the same seeded inputs.
 
 FP32 variant: stress and ice strength stored in FP32; strain rates, Delta, the stress
-update, the projection and the weak divergence computed in FP32, element integrals with
-Kahan-compensated sums; the velocity update, lumped-mass inversion, advection and the
-conservation sums stay FP64. Tolerance: 1e-5 relative L2 per field (u, v, h, A, sigma)
+update, the projection and the weak divergence computed in FP32, the relaxed stress
+sigma - sigma/alpha + projection summed with compensation and stored with stochastic
+rounding; the velocity update, lumped-mass inversion, advection and the conservation sums
+stay FP64. Tolerance: 1e-5 relative L2 per field (u, v, h, A, sigma)
 against the FP64 path.
 """
 import numpy as np
