@@ -832,15 +832,13 @@ __device__ __forceinline__ constexpr double invMassRef(int s)
 {
     return s == 0 ? 1.0 : (s <= 2 ? 12.0 : (s <= 4 ? 180.0 : (s == 5 ? 144.0 : 2160.0)));
 }
-// The relaxed stress sum: plain in FP64; in FP32 compensated (the exact sum is s - c;
-// initProduct and addProduct keep the rounding error of a product as well).
+// The relaxed stress: plain in FP64; in FP32 a (sum, compensation) pair whose exact value
+// is s - c (initSum: Fast2Sum of a + b, exact when |a| >= |b| or a = 0).
 template <class R> struct Acc {
     R s;
     __device__ __forceinline__ void init(R v) { s = v; }
-    __device__ __forceinline__ void initProduct(R a, R b) { s = a * b; }
     __device__ __forceinline__ void add(R x) { s += x; }
     __device__ __forceinline__ R get() const { return s; }
-    __device__ __forceinline__ double exact() const { return static_cast<double>(s); }
 };
 template <> struct Acc<float> {
     float s, c;
@@ -849,28 +847,12 @@ template <> struct Acc<float> {
         s = v;
         c = 0.0f;
     }
-    __device__ __forceinline__ void initProduct(float a, float b)
+    __device__ __forceinline__ void initSum(float a, float b)
     {
-        s = a * b;
-        c = -fmaf(a, b, -s);
-    }
-    __device__ __forceinline__ void add(float x)
-    {
-        const float y = x - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    // adds a * b with the rounding error of the product kept in the compensation
-    __device__ __forceinline__ void addProduct(float a, float b)
-    {
-        const float p = a * b;
-        const float err = fmaf(a, b, -p);
-        add(p);
-        c -= err;
+        s = a + b;
+        c = (s - a) - b;
     }
     __device__ __forceinline__ float get() const { return s - c; }
-    __device__ __forceinline__ double exact() const { return static_cast<double>(s) - static_cast<double>(c); }
 };
 
 // The mEVP stress relaxation sigma <- (1 - 1/alpha) sigma + ... amplifies a bias in the
@@ -878,8 +860,11 @@ template <> struct Acc<float> {
 // the same error every subcycle). The FP32 stress is therefore stored with unbiased
 // stochastic rounding: x is rounded away from zero with probability (|x| - |rz(x)|) / ulp,
 // the random bits a hash of (element, coefficient, subcycle), so runs are reproducible.
-// One full hash per element and subcycle; the coefficient's bits from it by a Weyl step
-// and one multiply-xorshift round.
+// One full hash per element and subcycle; a coefficient's bits from it by a Weyl step and
+// one multiply-xorshift round. (The hash XORed with a constant per coefficient is uniform
+// too, and as unbiased, but the roundings of one element's coefficients are then
+// correlated; measured on the FP32 variant's tests, that doubles to quadruples the stress
+// error against FP64.)
 __device__ __forceinline__ uint32_t elementSeed(uint32_t e, uint32_t it)
 {
     uint32_t h = e * 0x9E3779B1u ^ (it + 1u) * 0xC2B2AE3Du;
@@ -898,26 +883,26 @@ __device__ __forceinline__ uint32_t coefficientBits(uint32_t seed, uint32_t k)
     x ^= x >> 15;
     return x;
 }
-// For |x| in the normal FP32 range: the FP32 pattern of x truncated is bits 29-63 of its
-// FP64 magnitude with the exponent rebiased (1023 -> 127); adding a uniform 29-bit integer
-// below that cut first carries into the kept bits with probability (low 29 bits) / 2^29,
-// which is (|x| - |rz(x)|) / ulp exactly. Outside that range, round to nearest.
-__device__ __forceinline__ float roundStochastic(double x, uint32_t bits)
+// The exact value s - c of a pair with |c| at most half an ulp of s lies between s and its
+// neighbour g toward s - c; it is rounded to g with probability |c| / |g - s| (g - s is
+// exact), which is (|x| - |rz(x)|) / ulp of x = s - c rounded away from zero or toward it.
+// c = 0 (s exact, zero included) keeps s. The uniform u in [1, 2) takes 23 random bits as
+// the mantissa of 1.0; (u - 1) |g - s| is then exact.
+__device__ __forceinline__ float roundStochastic(const Acc<float>& a, uint32_t bits)
 {
-    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(x));
-    const uint64_t m = (b & 0x7fffffffffffffffull) + (bits & 0x1fffffffu);
-    const uint32_t sign = static_cast<uint32_t>(b >> 32) & 0x80000000u;
-    const uint32_t f = (static_cast<uint32_t>(m >> 29) - (896u << 23)) | sign;
-    const double ax = fabs(x);
-    return ax >= 0x1p-126 && ax < 0x1p128 ? __uint_as_float(f) : static_cast<float>(x);
+    const uint32_t sb = __float_as_uint(a.s);
+    const bool away = ((sb ^ __float_as_uint(a.c)) >> 31) != 0; // the exact value is farther from zero
+    const float g = __uint_as_float(away ? sb + 1u : sb - 1u);
+    const float u = __uint_as_float((bits & 0x007fffffu) | 0x3f800000u);
+    const float gap = fabsf(g - a.s);
+    return fmaf(u, gap, -gap) < fabsf(a.c) ? g : a.s;
 }
 template <class R, class C> __device__ __forceinline__ R storeValue(const Acc<C>& a, int e, int k, uint32_t it)
 {
     if constexpr (sizeof(R) == 8)
         return a.get();
     else
-        return roundStochastic(
-            a.exact(), coefficientBits(elementSeed(static_cast<uint32_t>(e), it), static_cast<uint32_t>(k)));
+        return roundStochastic(a, coefficientBits(elementSeed(static_cast<uint32_t>(e), it), static_cast<uint32_t>(k)));
 }
 __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
 __device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
@@ -1052,12 +1037,13 @@ __device__ __forceinline__ void elementAffineCore(
     }
     // The relaxed previous stress, read only now (3 S registers stay free through the strain).
     // FP64: (1 - 1/alpha) sigma + t, the order of MEVPStressUpdateStep.hpp:78-116. FP32:
-    // sigma - sigma/alpha + t in a compensated sum, with the same rounded 1/alpha that scales
-    // the new term, so that the fixed point is the projected sigma_VP exactly (a separately
-    // rounded 1 - 1/alpha would move it by up to 2^-24 alpha relative). The rounding inside t
-    // moves the fixed point by as much and is not amplified by alpha; a rounding of the
-    // relaxed sum itself would be, which the compensation (and the unbiased stochastic
-    // rounding of the stored result) prevents.
+    // sigma + d with d = t - sigma/alpha in one FMA (one rounding, and exactly zero at the
+    // fixed point t = sigma (1/alpha), with the same rounded 1/alpha that scales the new
+    // term; a separately rounded 1 - 1/alpha would move the fixed point by up to 2^-24 alpha
+    // relative), then sigma + d as an exact pair, which the stochastic rounding of the
+    // stored value rounds without bias. The rounding inside t and d is made on terms of
+    // size sigma_VP / alpha and moves the fixed point by as much, relative; a rounding of
+    // sigma + d itself would be made on sigma and amplified alpha-fold, which the pair avoids.
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         if constexpr (sizeof(C) == 8) {
@@ -1073,15 +1059,9 @@ __device__ __forceinline__ void elementAffineCore(
         } else {
             const C o1 = old(0, s), o2 = old(1, s), o3 = old(2, s);
             const C mInvAlpha = -static_cast<C>(a.invAlpha);
-            acc1[s].init(o1);
-            acc2[s].init(o2);
-            acc3[s].init(o3);
-            acc1[s].addProduct(o1, mInvAlpha);
-            acc2[s].addProduct(o2, mInvAlpha);
-            acc3[s].addProduct(o3, mInvAlpha);
-            acc1[s].add(t1[s]);
-            acc2[s].add(t2[s]);
-            acc3[s].add(t3[s]);
+            acc1[s].initSum(o1, fmaf(o1, mInvAlpha, t1[s]));
+            acc2[s].initSum(o2, fmaf(o2, mInvAlpha, t2[s]));
+            acc3[s].initSum(o3, fmaf(o3, mInvAlpha, t3[s]));
         }
     }
 }
