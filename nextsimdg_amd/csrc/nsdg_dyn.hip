@@ -1280,68 +1280,61 @@ __device__ __forceinline__ void divergenceLocal(const NodeArgs& a, int e, double
         if (a.sph)
             divergenceMetric<CG>(a, op, e, s11, s12, tx, ty);
 }
-// The same on a uniform (affine) mesh with literal reference-element values and sum
-// factorisation: stress at the Gauss points, the Jacobian-weighted fluxes
-// T = w |J| J^-T-rows applied to (s11, s12) and (s12, s22), then
-// tx_j = sum_q (PHI_xi[j](q) Tx_xi(q) + PHI_eta[j](q) Tx_eta(q)), factorised per Gauss row.
+// The same on a uniform (affine) mesh. With the constant Jacobian the divergence is linear
+// in the stress coefficients through two reference-element matrices,
+//   Dxi[j][s] = int PHI_j,xi PSI_s,  Deta[j][s] = int PHI_j,eta PSI_s   (over [0,1]^2),
+// tx = Dxi p1 + Deta q1 and ty = Dxi p2 + Deta q2, with p1 = |J| (xi_x s11 + xi_y s12),
+// q1 = |J| (eta_x s11 + eta_y s12), p2 = |J| (xi_x s12 + xi_y s22), q2 = |J| (eta_x s12 +
+// eta_y s22): the reference's Gauss-point form (divS1, divS2: ParametricMap.cpp:269-271) with the
+// quadrature carried out in the literals. The matrices are the Gauss rule's values (exact
+// for these polynomial degrees), folded to constants; entries that vanish analytically
+// are exact zeros and are skipped (40 of 72 nonzero per matrix for CG2).
+template <int CG> __device__ __forceinline__ constexpr double divReference(int d, int j, int s)
+{
+    constexpr int NG = gp1d(cg2dgstress(CG)), K = CG + 1;
+    const int jx = j % K, jy = j / K;
+    double v = 0.0;
+    for (int qy = 0; qy < NG; ++qy)
+        for (int qx = 0; qx < NG; ++qx) {
+            const double x = gauss_point(NG, qx), y = gauss_point(NG, qy);
+            const double dphi = d == 0 ? cg_1d_dx(CG, jx, x) * cg_1d(CG, jy, y) : cg_1d(CG, jx, x) * cg_1d_dx(CG, jy, y);
+            v += gauss_weight(NG, qx) * gauss_weight(NG, qy) * dphi * dg_basis(s, x, y);
+        }
+    return (v < 1e-13 && v > -1e-13) ? 0.0 : v;
+}
 // Contributions from the stress coefficients of one element (registers).
 template <int CG, class R>
 __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R* s11, const R* s12, const R* s22,
     R* tx, R* ty)
 {
-    constexpr int S = cg2dgstress(CG), NG = gp1d(S), N = cgdofs(CG), K = CG + 1;
-    const R xiX = static_cast<R>(a.xiX), etaX = static_cast<R>(a.etaX);
-    const R xiY = static_cast<R>(a.xiY), etaY = static_cast<R>(a.etaY);
-    R sx[N], sy[N];
+    constexpr int S = cg2dgstress(CG), N = cgdofs(CG);
+    const R xiX = static_cast<R>(a.xiX * a.detJ), etaX = static_cast<R>(a.etaX * a.detJ);
+    const R xiY = static_cast<R>(a.xiY * a.detJ), etaY = static_cast<R>(a.etaY * a.detJ);
+    R p1[S], q1[S], p2[S], q2[S];
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-        sx[j] = sy[j] = 0;
-#pragma unroll
-    for (int gy = 0; gy < NG; ++gy) {
-        R ax[K], bx[K], ay[K], by[K]; // per node column: sums over the Gauss row
-#pragma unroll
-        for (int jx = 0; jx < K; ++jx)
-            ax[jx] = bx[jx] = ay[jx] = by[jx] = 0;
-#pragma unroll
-        for (int gx = 0; gx < NG; ++gx) {
-            double b[8];
-            dgBasisAt(gauss_point(NG, gx) - 0.5, gauss_point(NG, gy) - 0.5, b);
-            R v11 = 0, v12 = 0, v22 = 0;
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                v11 += s11[s] * static_cast<R>(b[s]);
-                v12 += s12[s] * static_cast<R>(b[s]);
-                v22 += s22[s] * static_cast<R>(b[s]);
-            }
-            const R w = static_cast<R>(gauss_weight(NG, gx) * gauss_weight(NG, gy) * a.detJ);
-            // grad phi = (phi_xi xiX + phi_eta etaX, phi_xi xiY + phi_eta etaY)
-            const R fxXi = w * (xiX * v11 + xiY * v12), fxEta = w * (etaX * v11 + etaY * v12);
-            const R fyXi = w * (xiX * v12 + xiY * v22), fyEta = w * (etaX * v12 + etaY * v22);
-#pragma unroll
-            for (int jx = 0; jx < K; ++jx) {
-                const R ph = static_cast<R>(cg_1d(CG, jx, gauss_point(NG, gx)));
-                const R dp = static_cast<R>(cg_1d_dx(CG, jx, gauss_point(NG, gx)));
-                ax[jx] += dp * fxXi;  // d/dxi part, integrated along x
-                bx[jx] += ph * fxEta; // d/deta part
-                ay[jx] += dp * fyXi;
-                by[jx] += ph * fyEta;
-            }
-        }
-#pragma unroll
-        for (int jy = 0; jy < K; ++jy) {
-            const R ph = static_cast<R>(cg_1d(CG, jy, gauss_point(NG, gy)));
-            const R dp = static_cast<R>(cg_1d_dx(CG, jy, gauss_point(NG, gy)));
-#pragma unroll
-            for (int jx = 0; jx < K; ++jx) {
-                sx[jx + K * jy] += ph * ax[jx] + dp * bx[jx];
-                sy[jx + K * jy] += ph * ay[jx] + dp * by[jx];
-            }
-        }
+    for (int k = 0; k < S; ++k) {
+        p1[k] = xiX * s11[k] + xiY * s12[k];
+        q1[k] = etaX * s11[k] + etaY * s12[k];
+        p2[k] = xiX * s12[k] + xiY * s22[k];
+        q2[k] = etaX * s12[k] + etaY * s22[k];
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        tx[j] = sx[j];
-        ty[j] = sy[j];
+        R x = 0, y = 0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double dxi = divReference<CG>(0, j, k), deta = divReference<CG>(1, j, k);
+            if (dxi != 0.0) {
+                x += static_cast<R>(dxi) * p1[k];
+                y += static_cast<R>(dxi) * p2[k];
+            }
+            if (deta != 0.0) {
+                x += static_cast<R>(deta) * q1[k];
+                y += static_cast<R>(deta) * q2[k];
+            }
+        }
+        tx[j] = x;
+        ty[j] = y;
     }
 }
 
