@@ -633,6 +633,13 @@ def main():
                      "byte_model": "8 (9 S + 2 D) N_e + 8 * 17 N_n per subcycle x subcycles per launch "
                                    f"({per_launch[dom]})",
                      "avg_launch_ms": round(avg_ms, 5), "launches": rstats[dom][0],
+                     # achieved / frac are effective rates on the reference formulation's bytes:
+                     # the two-subcycle kernel reads and writes the state once per two
+                     # subcycles, so they can pass the peak; the physical HBM rate is counter_rate
+                     "note": "achieved and frac: the two-pass byte model of the reference formulation over the "
+                             "launch time (an effective rate; above 1 when the kernel needs fewer bytes than the "
+                             "model, as the two-subcycle kernel does); counter_rate and counter_frac: the HBM "
+                             "bytes the counters measure per launch over the same time",
                      # the one-pass kernels' own compulsory bytes (stress r+w, strength, 10 node
                      # values): a tighter model than the two-pass one above
                      "achieved_fused_model": round(fused_of[dom] / (avg_ms * 1e-3) / 1e9, 1),
