@@ -1302,40 +1302,42 @@ template <int CG> __device__ __forceinline__ constexpr double divReference(int d
         }
     return (v < 1e-13 && v > -1e-13) ? 0.0 : v;
 }
+// t = Dxi p + Deta q with p = xi_x a + xi_y b, q = eta_x a + eta_y b (|J| in the scalars).
+template <int CG, class R>
+__device__ __forceinline__ void divergenceRows(R xiX, R xiY, R etaX, R etaY, const R* a, const R* b, R* t)
+{
+    constexpr int S = cg2dgstress(CG), N = cgdofs(CG);
+    R p[S], q[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        p[k] = xiX * a[k] + xiY * b[k];
+        q[k] = etaX * a[k] + etaY * b[k];
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        R x = 0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const double dxi = divReference<CG>(0, j, k), deta = divReference<CG>(1, j, k);
+            if (dxi != 0.0)
+                x += static_cast<R>(dxi) * p[k];
+            if (deta != 0.0)
+                x += static_cast<R>(deta) * q[k];
+        }
+        t[j] = x;
+    }
+}
 // Contributions from the stress coefficients of one element (registers).
 template <int CG, class R>
 __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R* s11, const R* s12, const R* s22,
     R* tx, R* ty)
 {
-    constexpr int S = cg2dgstress(CG), N = cgdofs(CG);
+    constexpr int S = cg2dgstress(CG);
     const R xiX = static_cast<R>(a.xiX * a.detJ), etaX = static_cast<R>(a.etaX * a.detJ);
     const R xiY = static_cast<R>(a.xiY * a.detJ), etaY = static_cast<R>(a.etaY * a.detJ);
-    R p1[S], q1[S], p2[S], q2[S];
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        p1[k] = xiX * s11[k] + xiY * s12[k];
-        q1[k] = etaX * s11[k] + etaY * s12[k];
-        p2[k] = xiX * s12[k] + xiY * s22[k];
-        q2[k] = etaX * s12[k] + etaY * s22[k];
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        R x = 0, y = 0;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const double dxi = divReference<CG>(0, j, k), deta = divReference<CG>(1, j, k);
-            if (dxi != 0.0) {
-                x += static_cast<R>(dxi) * p1[k];
-                y += static_cast<R>(dxi) * p2[k];
-            }
-            if (deta != 0.0) {
-                x += static_cast<R>(deta) * q1[k];
-                y += static_cast<R>(deta) * q2[k];
-            }
-        }
-        tx[j] = x;
-        ty[j] = y;
-    }
+    // tx from (s11, s12), then ty from (s12, s22): half the intermediate registers live
+    divergenceRows<CG>(xiX, xiY, etaX, etaY, s11, s12, tx);
+    divergenceRows<CG>(xiX, xiY, etaX, etaY, s12, s22, ty);
 }
 
 // The momentum update with the terms that stay fixed over the subcycles of one timestep
@@ -1730,35 +1732,39 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
         }
     }
     __syncthreads();
-    // sub-step 1, the nodes of the inner elements (within the domain) into LDS
-    {
-        NodeFz in[NB];
-        bool live[NB], dir[NB];
+    // sub-step 1, the nodes of the inner elements (within the domain) into LDS, in batches
+    // of NBB nodes per thread (each batch's frozen terms loaded before its gathers)
+    constexpr int NBB = 2;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int i = threadIdx.x + b * kBlock;
+    for (int b0 = 0; b0 < NB; b0 += NBB) {
+        NodeFz in[NBB];
+        bool live[NBB], dir[NBB];
+#pragma unroll
+        for (int bb = 0; bb < NBB; ++bb) {
+            const int i = threadIdx.x + (b0 + bb) * kBlock;
             const int nix = x0 + i % T::NW, niy = y0 + i / T::NW;
-            live[b] = i < NWH && nix >= 0 && nix <= CG * a.nx && niy >= 0 && niy <= CG * a.ny;
+            live[bb] = b0 + bb < NB && i < NWH && nix >= 0 && nix <= CG * a.nx && niy >= 0 && niy <= CG * a.ny;
             const int n = nix + row * niy;
-            dir[b] = live[b] ? a.dirNode[n] != 0 : true;
-            if (live[b] && !dir[b])
-                loadFrozen(a, n, in[b]);
+            dir[bb] = live[bb] ? a.dirNode[n] != 0 : true;
+            if (live[bb] && !dir[bb])
+                loadFrozen(a, n, in[bb]);
         }
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (!live[b])
+        for (int bb = 0; bb < NBB; ++bb) {
+            const int b = b0 + bb;
+            if (!live[bb])
                 continue;
             const int i = threadIdx.x + b * kBlock;
             const int nix = x0 + i % T::NW, niy = y0 + i / T::NW;
             double un = 0.0, vn = 0.0;
-            if (!dir[b]) {
+            if (!dir[bb]) {
                 double dx = 0.0, dy = 0.0;
                 forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int exa, int eya, int j) {
                     const int le = (exa - ex0 + 2) + T::W1 * (eya - ey0 + 2);
                     dx -= static_cast<double>(contrib[le][j][0]);
                     dy -= static_cast<double>(contrib[le][j][1]);
                 });
-                momentumFrozen(a, in[b], dx, dy, un, vn);
+                momentumFrozen(a, in[bb], dx, dy, un, vn);
             }
             uS[i] = un;
             vS[i] = vn;

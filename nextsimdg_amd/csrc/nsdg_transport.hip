@@ -349,10 +349,16 @@ template <int DG, bool UNI> __device__ __forceinline__ void finishCell(const TrA
 // ---------------------------------------------------------------- DGTransportOperator
 // FINISH: apply Dirichlet, inverse mass and the stage here (no periodic segments);
 // otherwise leave the raw increment for the periodic and finishing passes.
+// One element per thread; the runs of kBlock elements go to the XCDs in contiguous chunks
+// (xcdTile), so the rows above and below a run, which its edge fluxes read, were loaded
+// into the same XCD's L2 by the neighbouring runs.
 template <int DG, bool UNI, bool FINISH> __global__ __launch_bounds__(kBlock) void k_transport_op(TrArgs a)
 {
     constexpr int G = gp1d(DG), Q = G * G, E = edgedofs(DG);
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int run = xcdTile(blockIdx.x, (a.ne + kBlock - 1) / kBlock);
+    if (run < 0)
+        return;
+    const int e = run * kBlock + threadIdx.x;
     if (e >= a.ne)
         return;
     const int nx = a.m.nx, ny = a.m.ny, ix = e % nx, iy = e / nx, ne = a.ne;
@@ -822,7 +828,7 @@ template <int DG> struct TransportImpl final : TransportBase {
     // One DGTransportOperator application fused with its stage combination.
     void stage(TrArgs a)
     {
-        const unsigned nb = blocksFor(mesh.ne);
+        const unsigned nb = xcdGrid(static_cast<int>(blocksFor(mesh.ne)));
         if (!mesh.hasPeriodic) {
             Timed t(prof, s, "transport_op");
             if (mesh.uniform)
@@ -847,9 +853,9 @@ template <int DG> struct TransportImpl final : TransportBase {
             checkLaunch("transport_periodic");
         }
         if (mesh.uniform)
-            k_transport_finish<DG, true><<<nb, kBlock, 0, s>>>(a);
+            k_transport_finish<DG, true><<<blocksFor(mesh.ne), kBlock, 0, s>>>(a);
         else
-            k_transport_finish<DG, false><<<nb, kBlock, 0, s>>>(a);
+            k_transport_finish<DG, false><<<blocksFor(mesh.ne), kBlock, 0, s>>>(a);
         checkLaunch("transport_finish");
     }
     // step_rk1 / step_rk2 / step_rk3 (DGTransport.cpp:514-552) on a device field. With
