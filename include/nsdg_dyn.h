@@ -182,16 +182,16 @@ int nsdg_dyn_apply_boundaries(nsdg_dyn* d);           /* applyBoundaries, CGDyna
 int nsdg_dyn_subcycles(nsdg_dyn* d, int n);
 /* Element-pass arithmetic: NSDG_PREC_FP64 (default) or NSDG_PREC_FP32. */
 int nsdg_dyn_set_precision(nsdg_dyn* d, int precision);
-/* How FP64 subcycles on uniform meshes without periodic segments are grouped into
- * passes (no counterpart in the reference: a performance control; the results are the
- * same bits either way). NSDG_PAIR_AUTO (default): two subcycles per pass from
- * NSDG_PAIR_MIN_ELEMENTS elements up, where reading the state once per two subcycles
- * outweighs the extra halo work; NSDG_PAIR_ALWAYS; NSDG_PAIR_NEVER: one per pass. The
- * last subcycle of a run always runs alone. */
+/* How subcycles on uniform meshes without periodic segments are grouped into passes (no
+ * counterpart in the reference: a performance control; the results agree to an ulp,
+ * within 1e-14 relative). NSDG_PAIR_AUTO (default): two subcycles per pass from
+ * NSDG_PAIR_MIN_ELEMENTS elements up (0: on every such mesh; half the launches on small
+ * meshes, the state read once per two subcycles on large ones); NSDG_PAIR_ALWAYS;
+ * NSDG_PAIR_NEVER: one per pass. The last subcycle of a run always runs alone. */
 #define NSDG_PAIR_AUTO 0
 #define NSDG_PAIR_ALWAYS 1
 #define NSDG_PAIR_NEVER 2
-#define NSDG_PAIR_MIN_ELEMENTS 300000
+#define NSDG_PAIR_MIN_ELEMENTS 0
 int nsdg_dyn_set_pairing(nsdg_dyn* d, int mode);
 /* Benchmark cyclone wind (BenchmarkAtmosphere.cpp:29-76) evaluated on the device at the
  * cell lower-left corners x = i L/nx, y = j L/ny for time t (seconds since the start),

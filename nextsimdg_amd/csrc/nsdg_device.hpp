@@ -396,6 +396,12 @@ __device__ __forceinline__ uint32_t after(uint32_t off, double x)
     return off;
 }
 template <class T> __device__ __forceinline__ T ldAt(const T* base, uint32_t off) { return fieldAt(base, off); }
+// A double2 as its two halves (adjacent: the compiler can issue one 16-byte load).
+__device__ __forceinline__ double2 ld2At(const double2* base, uint32_t off)
+{
+    const double* b = reinterpret_cast<const double*>(base);
+    return make_double2(ldAt(b, off), ldAt(b, off + 8u));
+}
 constexpr int kXCD = 8;
 __device__ __forceinline__ int xcdTile(int b, int ntiles)
 {

@@ -326,12 +326,12 @@ template <int CG, int GG> __global__ void k_lumped_mass(MeshView m, double* out)
 // Node tiles: a block owns the CG nodes of the lower-left 2x2 (CG2) node blocks of TX x TY
 // elements (the last tile row/column also owns the domain edge) and gathers the weak
 // divergence of the tile and a one-element left/bottom halo (HX x HY elements) from LDS.
-template <int CG, int TX_, int TY_> struct TileShape {
+template <int CG, int TX_, int TY_, int BLOCK = kBlock> struct TileShape {
     static constexpr int TX = TX_, TY = TY_;
     static constexpr int HX = TX + 1, HY = TY + 1, N = cgdofs(CG);
     static constexpr int CW = CG * TX;                        // owned node columns
-    static constexpr int TPR = CW < kBlock ? CW : kBlock;     // threads per node row
-    static constexpr int RS = kBlock / TPR;                   // node rows per block sweep
+    static constexpr int TPR = CW < BLOCK ? CW : BLOCK;       // threads per node row
+    static constexpr int RS = BLOCK / TPR;                    // node rows per block sweep
     static constexpr int NB = (CG * TY + RS - 1) / RS;        // nodes per thread per sweep
 };
 
@@ -860,11 +860,13 @@ template <> struct Acc<float> {
 // the same error every subcycle). The FP32 stress is therefore stored with unbiased
 // stochastic rounding: x is rounded away from zero with probability (|x| - |rz(x)|) / ulp,
 // the random bits a hash of (element, coefficient, subcycle), so runs are reproducible.
-// One full hash per element and subcycle; a coefficient's bits from it by a Weyl step and
-// one multiply-xorshift round. (The hash XORed with a constant per coefficient is uniform
-// too, and as unbiased, but the roundings of one element's coefficients are then
-// correlated; measured on the FP32 variant's tests, that doubles to quadruples the stress
-// error against FP64.)
+// One full hash per element and subcycle; from it, one multiply-xorshift round per two
+// coefficients (a Weyl step apart), whose low and high halves give each 16 random bits.
+// (The hash XORed with a constant per coefficient is uniform too, and as unbiased, but the
+// roundings of one element's coefficients are then correlated; measured on the FP32
+// variant's tests, that doubles to quadruples the stress error against FP64.) 16 bits
+// quantise the rounding probability to 2^-16, a bias of at most 2^-16 of the gap per
+// rounding: 0.02 ulp after the alpha-fold accumulation near a fixed point.
 __device__ __forceinline__ uint32_t elementSeed(uint32_t e, uint32_t it)
 {
     uint32_t h = e * 0x9E3779B1u ^ (it + 1u) * 0xC2B2AE3Du;
@@ -877,23 +879,23 @@ __device__ __forceinline__ uint32_t elementSeed(uint32_t e, uint32_t it)
 }
 __device__ __forceinline__ uint32_t coefficientBits(uint32_t seed, uint32_t k)
 {
-    uint32_t x = seed + (k + 1u) * 0x9E3779B9u;
+    uint32_t x = seed + ((k >> 1) + 1u) * 0x9E3779B9u;
     x ^= x >> 16;
     x *= 0x7feb352du;
     x ^= x >> 15;
-    return x;
+    return (k & 1u) ? x >> 16 : x & 0xffffu;
 }
 // The exact value s - c of a pair with |c| at most half an ulp of s lies between s and its
 // neighbour g toward s - c; it is rounded to g with probability |c| / |g - s| (g - s is
 // exact), which is (|x| - |rz(x)|) / ulp of x = s - c rounded away from zero or toward it.
-// c = 0 (s exact, zero included) keeps s. The uniform u in [1, 2) takes 23 random bits as
-// the mantissa of 1.0; (u - 1) |g - s| is then exact.
-__device__ __forceinline__ float roundStochastic(const Acc<float>& a, uint32_t bits)
+// c = 0 (s exact, zero included) keeps s. The uniform u in [1, 2) takes the 16 random
+// bits as the top of the mantissa of 1.0; (u - 1) |g - s| is then exact.
+__device__ __forceinline__ float roundStochastic(const Acc<float>& a, uint32_t bits16)
 {
     const uint32_t sb = __float_as_uint(a.s);
     const bool away = ((sb ^ __float_as_uint(a.c)) >> 31) != 0; // the exact value is farther from zero
     const float g = __uint_as_float(away ? sb + 1u : sb - 1u);
-    const float u = __uint_as_float((bits & 0x007fffffu) | 0x3f800000u);
+    const float u = __uint_as_float((bits16 << 7) | 0x3f800000u);
     const float gap = fabsf(g - a.s);
     return fmaf(u, gap, -gap) < fabsf(a.c) ? g : a.s;
 }
@@ -927,7 +929,7 @@ __device__ __forceinline__ void elementAffineCore(
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
     const size_t ne = a.ne;
-    const bool oc = a.ocean[e] != 0;
+    const bool oc = ldAt(a.ocean, static_cast<uint32_t>(e)) != 0;
     const R* const Pg = strengthOf<R>(a);
     C buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
     if (oc) {
@@ -1011,7 +1013,7 @@ __device__ __forceinline__ void elementAffineCore(
                 }
             }
             const C X = DM2 + C(1.25) * (g11 * g11 + g22 * g22) + C(1.50) * g11 * g22 + g12 * g12;
-            const C P = Pg[(gx + NG * gy) * ne + e];
+            const C P = ldAt(Pg + (gx + NG * gy) * ne, static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R)));
             const C c = (P * C(0.125)) * rsq(X); // P / (8 Delta)
             const C w = static_cast<C>(gauss_weight(NG, gx) * gauss_weight(NG, gy)) * static_cast<C>(a.invAlpha);
             const C r11 = w * (c * (C(5.0) * g11 + C(3.0) * g22) - C(0.5) * P);
@@ -1072,20 +1074,21 @@ __device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, A
     constexpr int S = MomLayout<CG>::S, K = CG + 1;
     const size_t ne = a.ne;
     const R* const sig = stressOf<R>(a);
+    const uint32_t eoff = static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R));
     elementAffineCore<CG, DG, R, C, VP>(
         a, e, acc1, acc2, acc3,
         [&](double* lu, double* lv) {
             const int ix = e % a.nx, iy = e / a.nx, row = CG * a.nx + 1;
-            const int cgi = CG * row * iy + CG * ix;
+            const uint32_t noff = static_cast<uint32_t>(CG * row * iy + CG * ix) * 8u;
 #pragma unroll
             for (int r = 0; r < K; ++r)
 #pragma unroll
                 for (int c = 0; c < K; ++c) {
-                    lu[c + K * r] = a.u[cgi + c + r * row];
-                    lv[c + K * r] = a.v[cgi + c + r * row];
+                    lu[c + K * r] = ldAt(a.u + c + r * row, noff);
+                    lv[c + K * r] = ldAt(a.v + c + r * row, noff);
                 }
         },
-        [&](int c, int k) -> C { return sig[(c * S + k) * ne + e]; });
+        [&](int c, int k) -> C { return ldAt(sig + (c * S + k) * ne, eoff); });
 }
 template <int CG, int DG, class R, class C = R, bool VP = false>
 __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
@@ -1374,6 +1377,15 @@ __device__ __forceinline__ void loadFrozen(const NodeArgs& a, int n, NodeFz& in)
     in.cf = a.cf[n];
     in.qf = a.qf[n];
 }
+// The same with scalar field bases and 32-bit byte offsets (node n < 2^29).
+__device__ __forceinline__ void loadFrozenAt(const NodeArgs& a, uint32_t n, NodeFz& in)
+{
+    in.u = ldAt(a.u, n * 8u);
+    in.v = ldAt(a.v, n * 8u);
+    in.ocn = ld2At(a.ocn, n * 16u);
+    in.cf = ld2At(a.cf, n * 16u);
+    in.qf = ld2At(a.qf, n * 16u);
+}
 __device__ __forceinline__ void momentumFrozen(const NodeArgs& a, const NodeFz& in, double dSx, double dSy, double& un,
     double& vn)
 {
@@ -1646,16 +1658,22 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_residual_f
 // k_mevp_fused, in the same order, from the same operands (a halo value identically in
 // every tile that computes it); the compiler may still fuse a multiply-add differently
 // in the two kernels, which moves a value by an ulp.
-template <int CG> struct PairTile {
-    static constexpr int TX = 13, TY = 13;
+// FP64: 13 x 13 owned elements on 256 threads (its LDS allows two blocks per CU). FP32,
+// bound by instruction issue and with half the LDS per value: 29 x 13 on 512 threads, 2.47
+// element updates per owned element and two subcycles instead of 2.67.
+template <int CG, class R> struct PairTile {
+    static constexpr int TX = sizeof(R) == 8 ? 13 : 29, TY = 13;
     static constexpr int W1 = TX + 3, H1 = TY + 3;          // sub-step 1 elements
     static constexpr int W2 = TX + 1, H2 = TY + 1;          // sub-step 2 elements
     static constexpr int NW = CG * W2 + 1, NH = CG * H2 + 1; // the CG nodes of those
     static constexpr int N = cgdofs(CG);
-    using Inner = TileShape<CG, TX, TY>;
+    static constexpr int BLOCK = W1 * H1;                    // one sub-step 1 element per thread
+    using Inner = TileShape<CG, TX, TY, BLOCK>;
 };
-static_assert(PairTile<2>::W1 * PairTile<2>::H1 == kBlock, "pair tile must give one element per thread");
-static_assert(PairTile<2>::Inner::HX == PairTile<2>::W2 && PairTile<2>::Inner::HY == PairTile<2>::H2,
+static_assert(PairTile<2, double>::BLOCK == kBlock && PairTile<2, float>::BLOCK == 2 * kBlock,
+    "pair tiles of 256 (FP64) and 512 (FP32) threads");
+static_assert(PairTile<2, double>::Inner::HX == PairTile<2, double>::W2
+        && PairTile<2, double>::Inner::HY == PairTile<2, double>::H2,
     "sub-step 2 is the one-pass tile of the owned elements");
 // mEVP node update (as MevpNode) with the velocity entering it from the LDS node window of
 // k_mevp_pair (origin x0, y0 in global node coordinates, width nw).
@@ -1670,9 +1688,9 @@ struct MevpNodeWindow {
         const int l = (nix - x0) + nw * (niy - y0);
         in.u = uS[l];
         in.v = vS[l];
-        in.ocn = a.ocn[n];
-        in.cf = a.cf[n];
-        in.qf = a.qf[n];
+        in.ocn = ld2At(a.ocn, static_cast<uint32_t>(n) * 16u);
+        in.cf = ld2At(a.cf, static_cast<uint32_t>(n) * 16u);
+        in.qf = ld2At(a.qf, static_cast<uint32_t>(n) * 16u);
     }
     __device__ __forceinline__ void update(const NodeArgs& a, const In& in, int, bool, double dx, double dy,
         double& un, double& vn) const
@@ -1685,11 +1703,11 @@ struct MevpNodeWindow {
 // indices iter and iter + 1, as two single passes round it; the divergence and sub-step 2
 // use the rounded values; the node update stays FP64).
 template <int CG, int DG, class R>
-__global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
+__global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArgs f)
 {
-    using T = PairTile<CG>;
+    using T = PairTile<CG, R>;
     constexpr int N = T::N, S = cg2dgstress(CG), K = CG + 1;
-    constexpr int NWH = T::NW * T::NH, NB = (NWH + kBlock - 1) / kBlock;
+    constexpr int NWH = T::NW * T::NH, NB = (NWH + T::BLOCK - 1) / T::BLOCK;
     // the sub-step contributions, and sigma^(k+1) parked during sub-step 2, in one array
     constexpr int CWORDS = T::W1 * T::H1 * N * 2, SWORDS = 3 * S * T::W2 * T::H2;
     __shared__ R contribRaw[CWORDS > SWORDS ? CWORDS : SWORDS];
@@ -1741,20 +1759,20 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
         bool live[NBB], dir[NBB];
 #pragma unroll
         for (int bb = 0; bb < NBB; ++bb) {
-            const int i = threadIdx.x + (b0 + bb) * kBlock;
+            const int i = threadIdx.x + (b0 + bb) * T::BLOCK;
             const int nix = x0 + i % T::NW, niy = y0 + i / T::NW;
             live[bb] = b0 + bb < NB && i < NWH && nix >= 0 && nix <= CG * a.nx && niy >= 0 && niy <= CG * a.ny;
             const int n = nix + row * niy;
             dir[bb] = live[bb] ? a.dirNode[n] != 0 : true;
             if (live[bb] && !dir[bb])
-                loadFrozen(a, n, in[bb]);
+                loadFrozenAt(a, static_cast<uint32_t>(n), in[bb]);
         }
 #pragma unroll
         for (int bb = 0; bb < NBB; ++bb) {
             const int b = b0 + bb;
             if (!live[bb])
                 continue;
-            const int i = threadIdx.x + b * kBlock;
+            const int i = threadIdx.x + b * T::BLOCK;
             const int nix = x0 + i % T::NW, niy = y0 + i / T::NW;
             double un = 0.0, vn = 0.0;
             if (!dir[bb]) {
@@ -1813,10 +1831,11 @@ __global__ __launch_bounds__(kBlock) void k_mevp_pair(FusedArgs f)
                 n22[k] = storeValue<R>(acc3[k], e, 2 * S + k, f.e.iter + 1);
             }
             if (lx >= 2 && ly >= 2) {
+                const uint32_t eoff = static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R));
                 if constexpr (sizeof(R) == 8)
-                    storeTensor<S>(f.sigOut, a.ne, e, n11, n12, n22);
+                    storeTensorAt<S>(f.sigOut, a.ne, eoff, n11, n12, n22);
                 else
-                    storeTensor<S>(f.sigFOut, a.ne, e, n11, n12, n22);
+                    storeTensorAt<S>(f.sigFOut, a.ne, eoff, n11, n12, n22);
             }
             if (a.ocean[e])
                 divergenceAffineFrom<CG, R>(a, n11, n12, n22, tx, ty);
@@ -3245,11 +3264,10 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         nn1 = static_cast<size_t>(mesh.nx + 1) * (mesh.ny + 1);
         // the kernels index CG nodes with 32-bit ints (the mesh checks elements and vertices)
         require(nn < (size_t(1) << 31), "too many CG nodes for 32-bit node indices", NSDG_ERR_ARG);
-        if (!mesh.uniform) {
-            // the one-pass kernel addresses element fields and CG nodes with 32-bit byte offsets
-            require(ne < (size_t(1) << 29), "general meshes need fewer than 2^29 elements", NSDG_ERR_ARG);
-            require(nn < (size_t(1) << 29), "general meshes need fewer than 2^29 CG nodes", NSDG_ERR_ARG);
-        }
+        // the one-pass kernels address element fields and CG nodes with 32-bit byte offsets
+        // from scalar field bases (2^29 CG2 nodes: a 11585^2 mesh, ~210 GB of FP64 state)
+        require(ne < (size_t(1) << 29), "meshes need fewer than 2^29 elements", NSDG_ERR_ARG);
+        require(nn < (size_t(1) << 29), "meshes need fewer than 2^29 CG nodes", NSDG_ERR_ARG);
         tr = makeTransport(DG, mesh, s, prof);
         tr->setScheme(2); // DynamicsKernel.hpp:61
         cg2dg.build(CG, DG, mesh, s);
@@ -3797,7 +3815,8 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         checkLaunch("strain");
     }
     // Two subcycles per pass: uniform meshes without periodic segments (FP64 or FP32), by default
-    // from NSDG_PAIR_MIN_ELEMENTS up, where it is faster per subcycle than k_mevp_fused
+    // from NSDG_PAIR_MIN_ELEMENTS up (every such mesh: faster per step than k_mevp_fused
+    // from 128^2 to 2048^2 except near 320^2, within 3.5% there)
     // (below, the state largely stays in the memory-side cache).
     bool pairSubcycles() const
     {
@@ -3814,13 +3833,17 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         f.sigFOut = sigFAlt.p;
         f.uOut = uAlt.p;
         f.vOut = vAlt.p;
-        using T = PairTile<CG>;
-        const unsigned grid = xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+        auto gridOf = [&](auto tile) {
+            using T = decltype(tile);
+            return xcdGrid(((mesh.nx + T::TX - 1) / T::TX) * ((mesh.ny + T::TY - 1) / T::TY));
+        };
         if (precision == NSDG_PREC_FP32) {
             require(sigF.n == 3 * S * ne && sigFAlt.n == sigF.n, "FP32 buffers not allocated", NSDG_ERR_STATE);
-            k_mevp_pair<CG, DG, float><<<grid, kBlock, 0, s>>>(f);
+            using T = PairTile<CG, float>;
+            k_mevp_pair<CG, DG, float><<<gridOf(T {}), T::BLOCK, 0, s>>>(f);
         } else {
-            k_mevp_pair<CG, DG, double><<<grid, kBlock, 0, s>>>(f);
+            using T = PairTile<CG, double>;
+            k_mevp_pair<CG, DG, double><<<gridOf(T {}), T::BLOCK, 0, s>>>(f);
         }
         checkLaunch("mevp_pair");
         if (precision == NSDG_PREC_FP32)

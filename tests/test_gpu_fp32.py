@@ -110,7 +110,9 @@ def test_fp32_needs_a_uniform_mesh():
     k.synchronize()
 
 
-@pytest.mark.parametrize("nx,ny", [(2, 2), (13, 13), (27, 27), (40, 29)])
+# The FP32 pair kernel owns 29 x 13 elements per tile: meshes smaller than a tile, tile
+# multiples (58 x 26) and one past them (61 x 29, 40 x 29).
+@pytest.mark.parametrize("nx,ny", [(2, 2), (13, 13), (27, 27), (40, 29), (58, 26), (61, 29)])
 def test_fp32_paired_subcycles_equal_single_passes(nx, ny):
     """The FP32 variant with two subcycles per pass (k_mevp_pair<float>, forced on) against
     one per pass (k_mevp_fused<float>) from the same state, nine subcycles. The pair
