@@ -142,9 +142,11 @@ int nsdg_bbm_create(int cg_degree, int dg_adv, const nsdg_params* params, const 
  * CGDynamicsKernel.cpp:23-51). Builds the mesh, its Dirichlet lists, the transport and
  * momentum operators on the device. Stress and strain start at zero. With spherical != 0
  * the coordinates are (longitude, latitude) in radians (the module converts degrees,
- * MEVPDynamics.cpp:46-48). Non-uniform and spherical meshes (fewer than 2^29 elements)
- * run the FP64 subcycle from per-element geometry tables; the FP32 variant stays with
- * uniform meshes. */
+ * MEVPDynamics.cpp:46-48). Meshes need fewer than 2^29 elements and 2^29 CG nodes (the
+ * kernels address fields with 32-bit byte offsets: up to a 11585^2 CG2 mesh, about 210 GB
+ * of FP64 state); larger ones are refused with NSDG_ERR_ARG. Non-uniform and spherical
+ * meshes run the FP64 subcycle from per-element geometry tables; the FP32 variant stays
+ * with uniform meshes. */
 int nsdg_dyn_initialise(nsdg_dyn* d, size_t nx, size_t ny, const double* coords, int spherical,
     const double* mask);
 /* Initialise from an explicitly built mesh (landmask, Dirichlet and periodic lists as set). */

@@ -130,16 +130,17 @@ def test_budget_closes_at_size(nx):
     assert tot["limiter_A"] < -1e-10 * a0, tot
 
 
-def test_general_mesh_node_count_limit_is_an_argument_error():
-    """The one-pass general-mesh kernels address CG nodes with 32-bit byte offsets, so a
-    non-uniform CG2 mesh with 2^29 nodes or more is refused (NSDG_ERR_ARG) at initialise."""
+@pytest.mark.parametrize("uniform", [False, True])
+def test_mesh_node_count_limit_is_an_argument_error(uniform):
+    """The one-pass kernels address CG nodes with 32-bit byte offsets, so a CG2 mesh with
+    2^29 nodes or more, uniform or not, is refused (NSDG_ERR_ARG) at initialise."""
     import nextsimdg_amd as nd
 
-    _say("node-count guard on a general mesh")
+    _say(f"node-count guard, uniform={uniform}")
     nx, ny = 16384, 8192  # (2 nx + 1)(2 ny + 1) = 536,920,065 > 2^29 CG2 nodes, 2^27 elements
     j, i = np.meshgrid(np.arange(ny + 1, dtype=np.float64), np.arange(nx + 1, dtype=np.float64), indexing="ij")
     coords = np.empty(((nx + 1) * (ny + 1), 2))
-    coords[:, 0] = (i * 100.0 + 3.0 * np.sin(j)).ravel()  # non-uniform: rows shifted unevenly
+    coords[:, 0] = (i * 100.0 + (0.0 if uniform else 3.0 * np.sin(j))).ravel()  # else rows shifted unevenly
     coords[:, 1] = (j * 100.0).ravel()
     del i, j
     g = nd.MEVPDynamicsKernel(2, 6)
