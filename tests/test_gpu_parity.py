@@ -502,12 +502,13 @@ def test_state_round_trip_and_dg0_reset():
 
 
 @pytest.mark.parametrize("cg,dg", [(2, 6), (1, 3)])
-@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 9), (13, 13), (14, 14), (26, 26), (27, 27), (40, 29), (31, 7)])
+@pytest.mark.parametrize("nx,ny", [(2, 7), (7, 2), (2, 2), (3, 9), (13, 13), (14, 14), (26, 26), (27, 27),
+                                   (40, 29), (31, 7)])
 def test_paired_subcycles_equal_single_passes(cg, dg, nx, ny):
-    """Two subcycles per pass (k_mevp_pair, forced on below its size threshold), against
-    one per pass (k_mevp_fused) on the same state, on meshes smaller than a pair tile
-    (13 x 13), exact tile multiples and one element past them, with land cells. Nine
-    subcycles: four pairs and the last one alone.
+    """Two subcycles per pass (k_mevp_pair, forced on), against one per pass (k_mevp_fused,
+    forced on) on the same state, on two-element-wide meshes (the smallest the library
+    takes), meshes smaller than a pair tile (13 x 13), exact tile multiples and one element
+    past them, with land cells. Nine subcycles: four pairs and the last one alone.
 
     The pair kernel runs the arithmetic of the single one in the same order, but the
     compiler may fuse a multiply-add differently in the two kernels: single values then
@@ -547,7 +548,8 @@ def test_paired_subcycles_equal_single_passes(cg, dg, nx, ny):
         k.synchronize()
         ks.append(k)
     a, a2, b = ks
-    assert np.abs(a.get_cg("u")).max() > 1e-3
+    if min(nx, ny) > 2:  # a developed state (two-wide meshes are mostly Dirichlet nodes)
+        assert np.abs(a.get_cg("u")).max() > 1e-3
 
     def fields(k):
         out = {n: k.get_cg(n) for n in ("u", "v")}
