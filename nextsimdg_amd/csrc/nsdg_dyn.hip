@@ -922,15 +922,16 @@ template <> __device__ __forceinline__ const float* strengthOf<float>(const Elem
 // used for the momentum residual and for the stress of a predicted velocity.
 // vel(lu, lv) gives the element's node velocities by node row (ocean elements only);
 // old(c, s) the previous stress, component c (11, 12, 22), coefficient s.
-template <int CG, int DG, class R, class C, bool VP, class VEL, class OLD>
+// str(q) gives the ice strength at Gauss point q (row-major, x fastest); by default it is
+// read from the strength table.
+template <int CG, int DG, class R, class C, bool VP, class VEL, class OLD, class STR>
 __device__ __forceinline__ void elementAffineCore(
-    const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3, const VEL& vel, const OLD& old)
+    const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3, const VEL& vel, const OLD& old, const STR& str)
 {
     using L = MomLayout<CG>;
     constexpr int S = L::S, NG = L::G, N = L::N, K = CG + 1;
     const size_t ne = a.ne;
     const bool oc = ldAt(a.ocean, static_cast<uint32_t>(e)) != 0;
-    const R* const Pg = strengthOf<R>(a);
     C buf[3 * S]; // ocean: u (N), v (N) by node row; land: e11, e12, e22 (S each)
     if (oc) {
         double lu[N], lv[N];
@@ -1013,7 +1014,7 @@ __device__ __forceinline__ void elementAffineCore(
                 }
             }
             const C X = DM2 + C(1.25) * (g11 * g11 + g22 * g22) + C(1.50) * g11 * g22 + g12 * g12;
-            const C P = ldAt(Pg + (gx + NG * gy) * ne, static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R)));
+            const C P = str(gx + NG * gy);
             const C c = (P * C(0.125)) * rsq(X); // P / (8 Delta)
             const C w = static_cast<C>(gauss_weight(NG, gx) * gauss_weight(NG, gy)) * static_cast<C>(a.invAlpha);
             const C r11 = w * (c * (C(5.0) * g11 + C(3.0) * g22) - C(0.5) * P);
@@ -1067,9 +1068,21 @@ __device__ __forceinline__ void elementAffineCore(
         }
     }
 }
-// The element update from the state in HBM (velocity a.u, a.v; stress of precision R).
-template <int CG, int DG, class R, class C, bool VP>
-__device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3)
+template <int CG, int DG, class R, class C, bool VP, class VEL, class OLD>
+__device__ __forceinline__ void elementAffineCore(
+    const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3, const VEL& vel, const OLD& old)
+{
+    const R* const Pg = strengthOf<R>(a);
+    const size_t ne = a.ne;
+    const uint32_t eoff = static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R));
+    elementAffineCore<CG, DG, R, C, VP>(
+        a, e, acc1, acc2, acc3, vel, old, [&](int q) -> C { return ldAt(Pg + q * ne, eoff); });
+}
+// The element update from the state in HBM (velocity a.u, a.v; stress of precision R);
+// str(q) as in elementAffineCore.
+template <int CG, int DG, class R, class C, bool VP, class STR>
+__device__ __forceinline__ void elementAffineCompute(
+    const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3, const STR& str)
 {
     constexpr int S = MomLayout<CG>::S, K = CG + 1;
     const size_t ne = a.ne;
@@ -1088,7 +1101,16 @@ __device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, A
                     lv[c + K * r] = ldAt(a.v + c + r * row, noff);
                 }
         },
-        [&](int c, int k) -> C { return ldAt(sig + (c * S + k) * ne, eoff); });
+        [&](int c, int k) -> C { return ldAt(sig + (c * S + k) * ne, eoff); }, str);
+}
+template <int CG, int DG, class R, class C, bool VP>
+__device__ __forceinline__ void elementAffineCompute(const ElemArgs& a, int e, Acc<C>* acc1, Acc<C>* acc2, Acc<C>* acc3)
+{
+    const R* const Pg = strengthOf<R>(a);
+    const size_t ne = a.ne;
+    const uint32_t eoff = static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R));
+    elementAffineCompute<CG, DG, R, C, VP>(
+        a, e, acc1, acc2, acc3, [&](int q) -> C { return ldAt(Pg + q * ne, eoff); });
 }
 template <int CG, int DG, class R, class C = R, bool VP = false>
 __device__ __forceinline__ void elementAffine(const ElemArgs& a, int e)
@@ -1726,6 +1748,17 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
     const bool inDomain = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
     const int e = inDomain ? ex + a.nx * ey : 0;
     R s11[S], s12[S], s22[S]; // sigma^(k+1) of this thread's element
+    // the element's Gauss-point strength, read once for both sub-steps
+    constexpr int Q = MomLayout<CG>::Q;
+    R pk[Q];
+    if (inDomain) {
+        const R* const Pg = strengthOf<R>(f.e);
+        const uint32_t eoff = static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R));
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            pk[q] = ldAt(Pg + q * f.e.ne, eoff);
+    }
+    const auto kept = [&](int q) -> R { return pk[q]; };
     {
         R tx[N], ty[N];
 #pragma unroll
@@ -1733,7 +1766,7 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
             tx[j] = ty[j] = 0;
         if (inDomain) {
             Acc<R> acc1[S], acc2[S], acc3[S];
-            elementAffineCompute<CG, DG, R, R, false>(f.e, e, acc1, acc2, acc3);
+            elementAffineCompute<CG, DG, R, R, false>(f.e, e, acc1, acc2, acc3, kept);
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 s11[k] = storeValue<R>(acc1[k], e, 0 * S + k, f.e.iter);
@@ -1822,7 +1855,7 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
                             lv[c + K * r] = vS[bx + c + T::NW * (by + r)];
                         }
                 },
-                [&](int c, int k) -> R { return sigS[(c * S + k) * NI + le2]; });
+                [&](int c, int k) -> R { return sigS[(c * S + k) * NI + le2]; }, kept);
             R n11[S], n12[S], n22[S];
 #pragma unroll
             for (int k = 0; k < S; ++k) {
