@@ -112,8 +112,9 @@ def test_fp32_needs_a_uniform_mesh():
 
 # The FP32 pair kernel owns 29 x 13 elements per tile: meshes smaller than a tile, tile
 # multiples (58 x 26) and one past them (61 x 29, 40 x 29).
+@pytest.mark.parametrize("cg,dg", [(2, 6), (1, 3)])
 @pytest.mark.parametrize("nx,ny", [(2, 2), (13, 13), (27, 27), (40, 29), (58, 26), (61, 29)])
-def test_fp32_paired_subcycles_equal_single_passes(nx, ny):
+def test_fp32_paired_subcycles_equal_single_passes(nx, ny, cg, dg):
     """The FP32 variant with two subcycles per pass (k_mevp_pair<float>, forced on) against
     one per pass (k_mevp_fused<float>) from the same state, nine subcycles. The pair
     kernel rounds each sub-step's stored stress stochastically with the same subcycle
@@ -124,8 +125,8 @@ def test_fp32_paired_subcycles_equal_single_passes(nx, ny):
     import nextsimdg_amd as nd
     from helpers import box_coords
 
-    _say(f"fp32 pairs vs single passes {nx}x{ny}")
-    cg, dg, S = 2, 6, 8
+    _say(f"fp32 pairs vs single passes cg={cg} dg={dg} {nx}x{ny}")
+    S = 8 if cg == 2 else 3
     coords = box_coords(nx, ny, 2000.0)
     mask = np.ones(nx * ny)
     if nx * ny >= 6:
