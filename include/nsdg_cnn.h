@@ -81,9 +81,12 @@ int nsdg_dyn_warm_start(nsdg_dyn* d, nsdg_cnn* net, double* r_predicted, double*
 /* Subcycles until r <= r_target or max_subcycles have run, in chunks of at most
  * check_every: once r has fallen over a chunk, the next chunk ends where its rate of
  * decrease reaches r_target (at least one subcycle); an r within 1% of r_target that did
- * not fall over the last chunk gets chunks of at most 5. */
-int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, int* done,
-    double* r_final);
+ * not fall over the last chunk gets chunks of at most 5. r_start: the residual of the
+ * current state when the caller has it (after nsdg_dyn_warm_start: r_predicted if the
+ * prediction was kept, else r_cold), which saves one residual evaluation; a negative
+ * value has it computed. */
+int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, double r_start,
+    int* done, double* r_final);
 
 #ifdef __cplusplus
 }

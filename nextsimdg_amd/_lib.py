@@ -119,7 +119,7 @@ SIGNATURES = {
     "nsdg_dyn_velocity_target": (_i, [_vp, _vp, _vp]),
     "nsdg_dyn_apply_velocity": (_i, [_vp, _vp]),
     "nsdg_dyn_warm_start": (_i, [_vp, _vp, _dp, _dp, ctypes.POINTER(_i)]),
-    "nsdg_dyn_subcycles_to_residual": (_i, [_vp, _d, _i, _i, ctypes.POINTER(_i), _dp]),
+    "nsdg_dyn_subcycles_to_residual": (_i, [_vp, _d, _i, _i, _d, ctypes.POINTER(_i), _dp]),
 }
 
 

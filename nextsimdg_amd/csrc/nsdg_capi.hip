@@ -693,10 +693,12 @@ int nsdg_dyn_warm_start(nsdg_dyn* d, nsdg_cnn* net, double* r_predicted, double*
             *used = keep ? 1 : 0;
     });
 }
-int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, int* done,
-    double* r_final)
+int nsdg_dyn_subcycles_to_residual(nsdg_dyn* d, double r_target, int max_subcycles, int check_every, double r_start,
+    int* done, double* r_final)
 {
-    return guard([&] { nonNull(d, "dyn")->k->subcyclesToResidual(r_target, max_subcycles, check_every, done, r_final); });
+    return guard([&] {
+        nonNull(d, "dyn")->k->subcyclesToResidual(r_target, max_subcycles, check_every, r_start, done, r_final);
+    });
 }
 
 } // extern "C"

@@ -3929,14 +3929,15 @@ template <int CG, int DG> struct DynImpl final : DynBase {
     // decrease reaches rTarget (at least one subcycle), so the run stops within a few
     // subcycles of the crossing instead of up to `every` past it; a residual within 1% of
     // the target that did not fall over the last chunk gets chunks of at most 5.
-    void subcyclesToResidual(double rTarget, int maxN, int every, int* done, double* rFinal) override
+    void subcyclesToResidual(double rTarget, int maxN, int every, double rStart, int* done, double* rFinal) override
     {
         need();
         needMEVP("the residual-driven subcycle loop");
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
         require(maxN >= 0 && every >= 1, "need max_subcycles >= 0 and check_every >= 1");
-        double r = 0.0;
-        residual(&r, nullptr, nullptr);
+        double r = rStart;
+        if (!(rStart >= 0.0))
+            residual(&r, nullptr, nullptr);
         int n = 0;
         if (r > rTarget && maxN > 0) {
             beginRun();

@@ -37,7 +37,7 @@ struct DynBase {
     virtual void residual(double* r, double* normR, double* normD) = 0;
     virtual void stressFromVelocity() = 0;
     virtual void setBrittle(const nsdg_meb_params& m) = 0;
-    virtual void subcyclesToResidual(double rTarget, int maxN, int every, int* done, double* rFinal) = 0;
+    virtual void subcyclesToResidual(double rTarget, int maxN, int every, double rStart, int* done, double* rFinal) = 0;
     // learned warm start (CG2 element grid; see include/nsdg_cnn.h)
     virtual int gridNx() const = 0;
     virtual int gridNy() const = 0;

@@ -172,10 +172,12 @@ def warm_start(k, net):
     return rp.value, rc.value, bool(used.value)
 
 
-def subcycles_to_residual(k, r_target, max_subcycles, check_every):
+def subcycles_to_residual(k, r_target, max_subcycles, check_every, r_start=-1.0):
+    """Subcycles until r <= r_target; r_start is the residual of the current state when
+    known (a negative value has it computed). Returns (subcycles run, final r)."""
     done, r = ctypes.c_int(), ctypes.c_double()
     check(k.lib.nsdg_dyn_subcycles_to_residual(k.h, float(r_target), int(max_subcycles), int(check_every),
-                                               ctypes.byref(done), ctypes.byref(r)))
+                                               float(r_start), ctypes.byref(done), ctypes.byref(r)))
     return done.value, r.value
 
 

@@ -84,3 +84,8 @@ def test_gate_keeps_the_better_start_and_restores_the_cold_one_exactly(trained):
     done, r = L.subcycles_to_residual(k, r_plain, 400, 10)
     _say(f"  plain r after 100: {r_plain:.4e}; gated start reaches {r:.4e} after {done}")
     assert r <= r_plain and done <= 110
+    # the same solve started from the residual the gate computed: the same subcycles, bits
+    k.load_state(blob)
+    rp2, rc2, used2 = L.warm_start(k, net)
+    assert (rp2, rc2, used2) == (rp, rc, used)
+    assert L.subcycles_to_residual(k, r_plain, 400, 10, rp2 if used2 else rc2) == (done, r)
