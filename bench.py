@@ -11,7 +11,6 @@ the 512 x 512 CG2/DG2 box (dx = 1 km, dt = 120 s). Prints one JSON line.
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 

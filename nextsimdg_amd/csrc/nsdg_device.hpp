@@ -187,7 +187,7 @@ template <int NF> __device__ void blockTwoSum(double (*sh)[kBlock], double* sum,
     }
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
+        if (threadIdx.x < static_cast<unsigned>(w)) {
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 double s, err;

@@ -1357,7 +1357,6 @@ template <int CG, class R>
 __device__ __forceinline__ void divergenceAffineFrom(const NodeArgs& a, const R* s11, const R* s12, const R* s22,
     R* tx, R* ty)
 {
-    constexpr int S = cg2dgstress(CG);
     const R xiX = static_cast<R>(a.xiX * a.detJ), etaX = static_cast<R>(a.etaX * a.detJ);
     const R xiY = static_cast<R>(a.xiY * a.detJ), etaY = static_cast<R>(a.etaY * a.detJ);
     // tx from (s11, s12), then ty from (s12, s22): half the intermediate registers live
@@ -2010,7 +2009,7 @@ __device__ __forceinline__ void elementGeomUpdate(const ElemArgs& a, const doubl
     double* s12, double* s22, const COL& col)
 {
     using L = GeoLayout<CG>;
-    constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
+    constexpr int S = L::S, NG = L::G;
     const size_t ne = a.ne;
     const uint32_t off = static_cast<uint32_t>(e) * 8u;
     // The strain of the velocity is computed for every element (the nodes of land elements
@@ -3040,7 +3039,7 @@ __device__ __forceinline__ void bbmElementGeom(const ElemArgs& a, const BrittleA
 {
     using L = GeoLayout<CG>;
     using BG = BrittleGeo<CG, DG>;
-    constexpr int S = L::S, NG = L::G, N = cgdofs(CG), K = CG + 1;
+    constexpr int S = L::S, NG = L::G;
     const size_t ne = a.ne;
     const uint32_t off = static_cast<uint32_t>(e) * 8u;
     BrittleArgs b = bIn;
