@@ -539,6 +539,51 @@ def config5_leg(nx, seed=5, spinup=2, check_every=100):
     return out
 
 
+MFMA_BF16_PEAK = 2.5e15  # dense BF16 FLOP/s of MI355X
+
+
+def network_leg(nx, reps=10):
+    """The warm-start network's forward pass at nx x nx (one image, 32 BF16 channels): the
+    3 x 3 convolution kernels' times (HIP events), their dense BF16 MFMA rate against the
+    2.5 PFLOP/s peak (2 x 32 x 9 x 32 FLOP per pixel and hidden layer; the output layer
+    computes one 16-channel MFMA tile, 2 x 32 x 9 x 16) and their HBM rate on the
+    activations they read and write (64 B in per pixel; 64 B out, 32 B for the FP32 output
+    layer)."""
+    import numpy as np
+    from nextsimdg_amd import learned as L
+
+    net = L.WarmStartNet(0)
+    if os.path.exists(WEIGHTS):
+        net.set_params(np.load(WEIGHTS))
+    rng = np.random.default_rng(0)
+    xf = rng.normal(0.0, 1.0, (1, nx, nx, L.CHANNELS)).astype(np.float32)
+    x = L.DeviceArray.from_numpy((xf.view(np.uint32) >> 16).astype(np.uint16))
+    y = L.DeviceArray((1, nx, nx, L.OUTPUTS), np.float32)
+    net.forward(x, y)
+    net.set_profiling(True)
+    for _ in range(reps):
+        net.forward(x, y)
+    npix = nx * nx
+    out = {"workload": f"warm-start network forward, {nx}x{nx} element grid, 4 layers of 3x3 convolutions, "
+                       "32 channels, BF16 on MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulation",
+           "kernels": {}}
+    for name, layers, cout, out_bytes in (("cnn_conv", 3, 32, 64), ("cnn_conv_out", 1, 16, 32)):
+        c, t = net.kernel_stats(name)
+        if not c:
+            continue
+        ms = t / c
+        flop = 2.0 * L.CHANNELS * 9 * cout * npix
+        out["kernels"][name] = {"layers_per_forward": layers, "avg_launch_ms": round(ms, 5),
+                                "tflops": round(flop / (ms * 1e-3) / 1e12, 1),
+                                "mfma_frac": round(flop / (ms * 1e-3) / MFMA_BF16_PEAK, 4),
+                                "hbm_gbs": round((64 + out_bytes) * npix / (ms * 1e-3) / 1e9, 1),
+                                "hbm_frac": round((64 + out_bytes) * npix / (ms * 1e-3) / HBM_PEAK, 4)}
+    c1, t1 = net.kernel_stats("cnn_conv")
+    c2, t2 = net.kernel_stats("cnn_conv_out")
+    out["forward_ms"] = round((t1 + t2) / reps, 4) if c1 and c2 else None
+    return out
+
+
 def calibrate_copy(mib=1024, reps=3):
     """A plain device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
     counters of rocprofv3 --pmc runs to bytes."""
@@ -572,6 +617,7 @@ def main():
     ap.add_argument("--spherical-nx", type=int, default=512, help="spherical patch size (0 skips it)")
     ap.add_argument("--budget-steps", type=int, default=10, help="mass budget steps at --nx (0 skips it)")
     ap.add_argument("--config5-nx", type=int, default=2048, help="config 5 (storms, leads) size (0 skips it)")
+    ap.add_argument("--network-nx", type=int, default=1024, help="network forward size (0 skips it)")
     ap.add_argument("--no-config1", action="store_true", help="skip config 1 (128^2 CG1/DG1, CPU and GPU)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="measured HBM bytes per launch (from rocprofv3 --pmc) to report as traffic")
@@ -679,6 +725,9 @@ def main():
     if args.config5_nx:
         log(f"bench: config 5 (storms, leads) at {args.config5_nx}^2 ...")
         out["config5"] = config5_leg(args.config5_nx)
+    if args.network_nx:
+        log(f"bench: network forward at {args.network_nx}^2 ...")
+        out["network"] = network_leg(args.network_nx)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         try:

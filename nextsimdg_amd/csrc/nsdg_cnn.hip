@@ -58,62 +58,152 @@ enum Epi { EPI_RELU = 0, EPI_OUT = 1, EPI_MASK = 2 };
 //   EPI_RELU: out16[p][co] = bf16(max(acc + bias, 0))          (32 channels)
 //   EPI_OUT:  out32[p][co] = acc + bias, co < 8                 (first 16 channels computed)
 //   EPI_MASK: out16[p][co] = bf16(acc) where mask[p][co] > 0, else 0
+// A block's band is 64 pixels (16 per wave) of ROWS consecutive output rows. Its input,
+// rows y0 - 1 .. y0 + ROWS and columns x0 - 1 .. x0 + 64 with zeros outside the image,
+// is staged in LDS once (16-byte chunks, coalesced), at a pixel stride of 80 bytes so that
+// the 16 pixels of a fragment read fall in distinct banks. Input row y0 + r is read once
+// per dx and feeds the output rows y0 + r - dy of its three taps; each output still sums
+// its taps in tap order (dy, then dx). The outputs go back through LDS and leave as
+// 16-byte stores of whole pixels. The next band's input is loaded into registers while
+// this one computes.
+constexpr int kConvRows = 8, kBandX = 16 * kWaves, kInRows = kConvRows + 2, kInCols = kBandX + 2;
+constexpr int kPix = 80; // bytes per staged input pixel (64 used)
+constexpr int kConvLds = kInRows * kInCols * kPix;
+static_assert(kConvRows * kBandX * C * 2 <= kConvLds, "the output stage fits in the input stage");
 template <int EPI>
 __global__ __launch_bounds__(kThreads) void k_conv(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
     const float* __restrict__ bias, const uint16_t* __restrict__ mask, uint16_t* __restrict__ out16,
     float* __restrict__ out32, int n, int ny, int nx)
 {
-    constexpr int NT = EPI == EPI_OUT ? 1 : 2;
-    const int lane = threadIdx.x % 64;
-    const long wave = (static_cast<long>(blockIdx.x) * kThreads + threadIdx.x) / 64;
-    const long nwaves = static_cast<long>(gridDim.x) * kWaves;
+    constexpr int NT = EPI == EPI_OUT ? 1 : 2, ROWS = kConvRows;
+    __shared__ __align__(16) unsigned char lds[kConvLds];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
     bf16x8 wr[TAPS][NT];
 #pragma unroll
     for (int t = 0; t < TAPS; ++t)
 #pragma unroll
         for (int j = 0; j < NT; ++j)
             wr[t][j] = frag(w + ((t * 2 + j) * 64 + lane) * 8);
-    const int tilesX = nx / 16;
-    const long ntiles = static_cast<long>(n) * ny * tilesX;
-    for (long tile = wave; tile < ntiles; tile += nwaves) {
-        const int tx = static_cast<int>(tile % tilesX);
-        const long rowId = tile / tilesX; // img * ny + y
-        const int y = static_cast<int>(rowId % ny);
-        const long img = rowId / ny;
-        const int x0 = 16 * tx;
-        floatx4 acc[NT];
+    const int bandsX = (nx + kBandX - 1) / kBandX, bandsY = (ny + ROWS - 1) / ROWS;
+    const long nbands = static_cast<long>(n) * bandsY * bandsX;
+    // The input chunks of a band, loaded into registers one band ahead: thread t holds
+    // chunks t, t + kThreads, ... (4 chunks of 16 bytes per pixel).
+    constexpr int NC = (kInRows * kInCols * 4 + kThreads - 1) / kThreads;
+    uint4 pre[NC];
+    const auto load = [&](long band) {
+        const int bx0 = static_cast<int>(band % bandsX) * kBandX;
+        const long rb = band / bandsX;
+        const int by0 = static_cast<int>(rb % bandsY) * ROWS;
+        const long bimg = rb / bandsY;
 #pragma unroll
-        for (int j = 0; j < NT; ++j)
-            acc[j] = floatx4 { 0.f, 0.f, 0.f, 0.f };
+        for (int k = 0; k < NC; ++k) {
+            const int c = threadIdx.x + k * kThreads;
+            const int q = c % 4, col = (c / 4) % kInCols, row = c / (4 * kInCols);
+            const int xp = bx0 - 1 + col, yp = by0 - 1 + row;
+            pre[k] = make_uint4(0, 0, 0, 0);
+            if (c < kInRows * kInCols * 4 && xp >= 0 && xp < nx && yp >= 0 && yp < ny)
+                pre[k] = *reinterpret_cast<const uint4*>(in + ((bimg * ny + yp) * nx + xp) * C + 8 * q);
+        }
+    };
+    if (blockIdx.x < nbands)
+        load(blockIdx.x);
+    for (long band = blockIdx.x; band < nbands; band += gridDim.x) {
+        const int x0 = static_cast<int>(band % bandsX) * kBandX;
+        const long rowBand = band / bandsX; // img * bandsY + by
+        const int y0 = static_cast<int>(rowBand % bandsY) * ROWS;
+        const long img = rowBand / bandsY;
+        // stage the input band, then start loading the next one
 #pragma unroll
-        for (int t = 0; t < TAPS; ++t) {
-            const int dy = t / 3 - 1, dx = t % 3 - 1;
-            const int xp = x0 + lane % 16 + dx, yp = y + dy;
-            uint4 av = make_uint4(0, 0, 0, 0);
-            if (xp >= 0 && xp < nx && yp >= 0 && yp < ny)
-                av = *reinterpret_cast<const uint4*>(in + ((img * ny + yp) * nx + xp) * C + 8 * (lane / 16));
-            const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+        for (int k = 0; k < NC; ++k) {
+            const int c = threadIdx.x + k * kThreads;
+            if (c < kInRows * kInCols * 4) {
+                const int q = c % 4, col = (c / 4) % kInCols, row = c / (4 * kInCols);
+                *reinterpret_cast<uint4*>(lds + (row * kInCols + col) * kPix + 16 * q) = pre[k];
+            }
+        }
+        __syncthreads();
+        if (band + gridDim.x < nbands)
+            load(band + gridDim.x);
+        floatx4 acc[ROWS][NT];
+#pragma unroll
+        for (int i = 0; i < ROWS; ++i)
 #pragma unroll
             for (int j = 0; j < NT; ++j)
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wr[t][j], acc[j], 0, 0, 0);
-        }
+                acc[i][j] = floatx4 { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            const int co = 16 * j + lane % 16;
+        for (int r = -1; r <= ROWS; ++r) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const long p = rowId * nx + x0 + 4 * (lane / 16) + i;
-                const float val = acc[j][i];
-                if constexpr (EPI == EPI_RELU) {
-                    out16[p * C + co] = toBF16(fmaxf(val + bias[co], 0.0f));
-                } else if constexpr (EPI == EPI_OUT) {
-                    if (co < OUT)
-                        out32[p * OUT + co] = val + bias[co];
-                } else {
-                    out16[p * C + co] = fromBF16(mask[p * C + co]) > 0.0f ? toBF16(val) : uint16_t(0);
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int col = 16 * wv + lane % 16 + dx + 1;
+                const bf16x8 a = __builtin_bit_cast(bf16x8,
+                    *reinterpret_cast<const uint4*>(lds + ((r + 1) * kInCols + col) * kPix + 16 * (lane / 16)));
+#pragma unroll
+                for (int dy = 1; dy >= -1; --dy) { // output row y0 + r - dy, tap (dy, dx)
+                    const int i = r - dy;
+                    if (i < 0 || i >= ROWS)
+                        continue;
+                    const int t = (dy + 1) * 3 + (dx + 1);
+#pragma unroll
+                    for (int j = 0; j < NT; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wr[t][j], acc[i][j], 0, 0, 0);
                 }
             }
         }
+        __syncthreads(); // every fragment read before the outputs overwrite the stage
+        // D[m][n]: lane holds channel n = lane % 16 (+ 16 j) of pixels m = 4 (lane / 16) + i
+        if constexpr (EPI == EPI_OUT) {
+            float* st = reinterpret_cast<float*>(lds); // [ROWS][kBandX][OUT]
+            const int co = lane % 16;
+            if (co < OUT) {
+#pragma unroll
+                for (int ir = 0; ir < ROWS; ++ir)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        st[(ir * kBandX + 16 * wv + 4 * (lane / 16) + i) * OUT + co] = acc[ir][0][i] + bias[co];
+            }
+        } else {
+            uint16_t* st = reinterpret_cast<uint16_t*>(lds); // [ROWS][kBandX][C]
+#pragma unroll
+            for (int ir = 0; ir < ROWS; ++ir)
+#pragma unroll
+                for (int j = 0; j < NT; ++j) {
+                    const int co = 16 * j + lane % 16;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float val = acc[ir][j][i];
+                        st[(ir * kBandX + 16 * wv + 4 * (lane / 16) + i) * C + co]
+                            = EPI == EPI_RELU ? toBF16(fmaxf(val + bias[co], 0.0f)) : toBF16(val);
+                    }
+                }
+        }
+        __syncthreads();
+        // whole pixels out: 16-byte chunks (4 per BF16 pixel, 2 per FP32 output pixel)
+        constexpr int CH = EPI == EPI_OUT ? 2 : 4;
+        for (int c = threadIdx.x; c < ROWS * kBandX * CH; c += kThreads) {
+            const int q = c % CH, px = (c / CH) % kBandX, ir = c / (CH * kBandX);
+            const int x = x0 + px, y = y0 + ir;
+            if (x >= nx || y >= ny)
+                continue;
+            const long p = (img * ny + y) * nx + x;
+            const uint4 v = *reinterpret_cast<const uint4*>(lds + (static_cast<size_t>(ir * kBandX + px) * CH + q) * 16);
+            if constexpr (EPI == EPI_OUT) {
+                *reinterpret_cast<uint4*>(out32 + p * OUT + 4 * q) = v;
+            } else if constexpr (EPI == EPI_RELU) {
+                *reinterpret_cast<uint4*>(out16 + p * C + 8 * q) = v;
+            } else { // keep the gradient where the layer's ReLU was active
+                const uint4 mk = *reinterpret_cast<const uint4*>(mask + p * C + 8 * q);
+                const uint32_t vv[4] = { v.x, v.y, v.z, v.w }, mm[4] = { mk.x, mk.y, mk.z, mk.w };
+                uint32_t o[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t lo = fromBF16(static_cast<uint16_t>(mm[k] & 0xffffu)) > 0.0f ? (vv[k] & 0xffffu) : 0u;
+                    const uint32_t hi = fromBF16(static_cast<uint16_t>(mm[k] >> 16)) > 0.0f ? (vv[k] & 0xffff0000u) : 0u;
+                    o[k] = lo | hi;
+                }
+                *reinterpret_cast<uint4*>(out16 + p * C + 8 * q) = make_uint4(o[0], o[1], o[2], o[3]);
+            }
+        }
+        __syncthreads(); // the stage is free for the next band
     }
 }
 
@@ -355,8 +445,8 @@ void conv(hipStream_t s, Profiler& prof, const char* name, const uint16_t* in, c
     const uint16_t* mask, uint16_t* out16, float* out32, int n, int ny, int nx)
 {
     Timed t(prof, s, name);
-    const long tiles = static_cast<long>(n) * ny * (nx / 16);
-    k_conv<EPI><<<gridFor(tiles, kWaves * 4, 8192), kThreads, 0, s>>>(in, w, bias, mask, out16, out32, n, ny, nx);
+    const long bands = static_cast<long>(n) * ((ny + kConvRows - 1) / kConvRows) * ((nx + kBandX - 1) / kBandX);
+    k_conv<EPI><<<gridFor(bands, 1, 8192), kThreads, 0, s>>>(in, w, bias, mask, out16, out32, n, ny, nx);
     checkLaunch(name);
 }
 } // namespace
