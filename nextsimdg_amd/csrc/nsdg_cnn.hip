@@ -324,14 +324,25 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(const uint16_t* __restrict__
     if (threadIdx.x < C)
         P[WSZ + threadIdx.x] = db;
 }
-// Sum of the partials in block order, into the parameter layout [co][ci][tap], bias.
+// Sum of the partials in block order, into the parameter layout [co][ci][tap], bias. The
+// loads go out 16 at a time ahead of their (in-order) additions.
 __global__ void k_wgrad_reduce(const float* part, int nb, float* g)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= PSZ)
         return;
     float sum = 0.0f;
-    for (int b = 0; b < nb; ++b)
+    int b = 0;
+    for (; b + 16 <= nb; b += 16) {
+        float v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            v[k] = part[static_cast<size_t>(b + k) * PSZ + e];
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            sum += v[k];
+    }
+    for (; b < nb; ++b)
         sum += part[static_cast<size_t>(b) * PSZ + e];
     if (e < WSZ) {
         const int t = e / (C * C), ci = (e / C) % C, co = e % C;
