@@ -14,6 +14,11 @@ the VP law, where any strain-rate noise well above DeltaMin = 2e-9 1/s turns int
 stress, so a prediction that is close in L2 can still carry a larger residual than u0.
 The gate then restores the cold start exactly, and the residual-targeted solve from it
 is the plain one.
+
+The stress half of a warm start has the same fragility: started from the converged
+stress with the cold velocity, mEVP goes below the plain path's residual within a few
+subcycles, but the same stress with an independent smooth error of 1% of the stress
+change makes the start several times worse than the cold one.
 """
 import numpy as np
 import pytest
@@ -89,3 +94,58 @@ def test_gate_keeps_the_better_start_and_restores_the_cold_one_exactly(trained):
     rp2, rc2, used2 = L.warm_start(k, net)
     assert (rp2, rc2, used2) == (rp, rc, used)
     assert L.subcycles_to_residual(k, r_plain, 400, 10, rp2 if used2 else rc2) == (done, r)
+
+
+def _smooth(a, nx, passes):
+    g = a.reshape(nx, nx, -1).copy()
+    for _ in range(passes):
+        p = np.pad(g, ((1, 1), (1, 1), (0, 0)), mode="edge")
+        g = (p[1:-1, 1:-1] + p[:-2, 1:-1] + p[2:, 1:-1] + p[1:-1, :-2] + p[1:-1, 2:]) / 5
+    return g.reshape(a.shape)
+
+
+def test_stress_start_reaches_the_plain_residual_in_a_few_subcycles_unless_perturbed():
+    from nextsimdg_amd import learned as L
+
+    nx, names = 256, ("s11", "s12", "s22")
+    k, off = L.new_track(nx, 12, "heterogeneous")
+    for s in range(2):
+        L.plain_step(k, off, s)
+    blob = k.save_state()
+
+    def start(stress=None):
+        k.load_state(blob)
+        L.begin_step(k, off, 2)
+        if stress is not None:
+            for n, a in zip(names, stress):
+                k.setData(n, a, a.shape[1])
+
+    start()
+    r_cold = k.momentum_residual()[0]
+    s0 = [k.getDGData(n).copy() for n in names]
+    k.subcycles(100)
+    r_plain = k.momentum_residual()[0]
+    start()
+    done, r_conv = L.subcycles_to_residual(k, 0.05, 10000, 1000)
+    s_conv = [k.getDGData(n).copy() for n in names]
+    _say(f"  r cold {r_cold:.4e}, after 100 plain subcycles {r_plain:.4e}, converged {r_conv:.4e} after {done}")
+    assert r_conv <= 0.05
+
+    def r_after_5(stress):
+        start(stress)
+        assert k.momentum_residual()[0] == pytest.approx(r_cold, rel=1e-12)  # it sees the velocity only
+        k.subcycles(5)
+        return k.momentum_residual()[0]
+
+    r_exact = r_after_5(s_conv)
+    r_blend = r_after_5([a + 0.25 * (b - a) for a, b in zip(s0, s_conv)])
+    rng = np.random.default_rng(0)
+    noisy = []
+    for a, b in zip(s0, s_conv):
+        z = _smooth(rng.standard_normal(a.shape), nx, 8)
+        noisy.append(b + 0.01 * np.sqrt(np.mean((b - a) ** 2)) * z / np.sqrt(np.mean(z ** 2)))
+    r_noisy = r_after_5(noisy)
+    _say(f"  r after 5 subcycles from: converged stress {r_exact:.4e}, a quarter of the way {r_blend:.4e}, "
+         f"converged + 1% smooth error {r_noisy:.4e}")
+    assert r_exact < r_plain and r_blend < r_plain
+    assert r_noisy > 2 * r_cold
