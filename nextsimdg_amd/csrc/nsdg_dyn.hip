@@ -704,20 +704,17 @@ __device__ __forceinline__ double iceStrength(double Pstar, double hg, double ag
     ag = fmin(fmax(ag, 0.0), 1.0);
     return Pstar * hg * exp(-20.0 * (1.0 - ag));
 }
-// The strength depends only on h and A, which the subcycles do not change: evaluated once
-// per timestep at the Gauss points of every element, Pg[q * ne + e], q = gx + G gy.
-template <int CG, int DG, class T>
-__global__ __launch_bounds__(kBlock) void k_strength(int ne, const double* h, const double* A, double Pstar, T* Pg)
+// The strength of element e at its Gauss points, pk[gx + G gy], from its h and A.
+template <int CG, int DG>
+__device__ __forceinline__ void gaussStrength(const double* h, const double* A, size_t ne, int e, double Pstar,
+    double* pk)
 {
     constexpr int NG = MomLayout<CG>::G;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne)
-        return;
     double hc[DG], ac[DG];
 #pragma unroll
     for (int i = 0; i < DG; ++i) {
-        hc[i] = h[static_cast<size_t>(i) * ne + e];
-        ac[i] = A[static_cast<size_t>(i) * ne + e];
+        hc[i] = h[i * ne + e];
+        ac[i] = A[i * ne + e];
     }
 #pragma unroll
     for (int gy = 0; gy < NG; ++gy)
@@ -731,8 +728,23 @@ __global__ __launch_bounds__(kBlock) void k_strength(int ne, const double* h, co
                 hg += hc[i] * b[i];
                 ag += ac[i] * b[i];
             }
-            Pg[static_cast<size_t>(gx + NG * gy) * ne + e] = static_cast<T>(iceStrength(Pstar, hg, ag));
+            pk[gx + NG * gy] = iceStrength(Pstar, hg, ag);
         }
+}
+// The strength depends only on h and A, which the subcycles do not change: evaluated once
+// per timestep at the Gauss points of every element, Pg[q * ne + e], q = gx + G gy.
+template <int CG, int DG, class T>
+__global__ __launch_bounds__(kBlock) void k_strength(int ne, const double* h, const double* A, double Pstar, T* Pg)
+{
+    constexpr int Q = MomLayout<CG>::Q;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    double pk[Q];
+    gaussStrength<CG, DG>(h, A, ne, e, Pstar, pk);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        Pg[static_cast<size_t>(q) * ne + e] = static_cast<T>(pk[q]);
 }
 
 // Element-wise conversion between the FP64 stress and its FP32 copy.
@@ -1561,35 +1573,63 @@ template <int CG, int DG, class R> __global__ __launch_bounds__(kBlock) void k_m
         a, [&](int le, int j, int c) { return contrib[le][j][c]; }, ex0, ey0, f.uOut, f.vOut);
 }
 
-// R_x^2 + R_y^2 and (M_L A tau_a)^2 of node n, added to r2 and d2.
-__device__ __forceinline__ void residualNode(const NodeArgs& a, int n, double dSx, double dSy, double& r2, double& d2)
+// The node values the momentum residual reads.
+struct NodeRes {
+    double u, v, uo, vo, ua, va, H, A, ML, u0, v0, gx, gy;
+};
+__device__ __forceinline__ void loadRes(const NodeArgs& a, int n, NodeRes& x)
 {
-    const double u = a.u[n], v = a.v[n], uo = a.uo[n], vo = a.vo[n], ua = a.ua[n], va = a.va[n];
-    const double H = a.H[n], A = a.A[n], ML = a.ML[n];
+    x.u = a.u[n];
+    x.v = a.v[n];
+    x.uo = a.uo[n];
+    x.vo = a.vo[n];
+    x.ua = a.ua[n];
+    x.va = a.va[n];
+    x.H = a.H[n];
+    x.A = a.A[n];
+    x.ML = a.ML[n];
+    x.u0 = a.u0[n];
+    x.v0 = a.v0[n];
+    x.gx = a.gx[n];
+    x.gy = a.gy[n];
+}
+// R_x^2 + R_y^2 and (M_L A tau_a)^2 of a node with values x, added to r2 and d2.
+__device__ __forceinline__ void residualFrom(const NodeArgs& a, const NodeRes& x, double dSx, double dSy, double& r2,
+    double& d2)
+{
+    const double u = x.u, v = x.v, uo = x.uo, vo = x.vo, ua = x.ua, va = x.va;
+    const double H = x.H, A = x.A, ML = x.ML;
     const double absatm = sqrt(ua * ua + va * va);
     const double du = uo - u, dv = v - vo;
     const double absocn = sqrt(du * du + dv * dv);
     const double c1 = a.rho_ice * H / a.dt, rhoH = a.rho_ice * H;
     const double tax = a.F_atm * absatm * ua, tay = a.F_atm * absatm * va;
     const double rx = ML
-            * (c1 * (u - a.u0[n]) - A * (tax + a.F_ocean * absocn * (uo - u)) + rhoH * a.fc * u
-                + rhoH * a.gravity * a.gx[n])
+            * (c1 * (u - x.u0) - A * (tax + a.F_ocean * absocn * (uo - u)) + rhoH * a.fc * u
+                + rhoH * a.gravity * x.gx)
         - dSx;
     const double ry = ML
-            * (c1 * (v - a.v0[n]) - A * (tay + a.F_ocean * absocn * (vo - v)) - rhoH * a.fc * v
-                + rhoH * a.gravity * a.gy[n])
+            * (c1 * (v - x.v0) - A * (tay + a.F_ocean * absocn * (vo - v)) - rhoH * a.fc * v
+                + rhoH * a.gravity * x.gy)
         - dSy;
     const double fx = ML * A * tax, fy = ML * A * tay;
     r2 += rx * rx + ry * ry;
     d2 += fx * fx + fy * fy;
+}
+__device__ __forceinline__ void residualNode(const NodeArgs& a, int n, double dSx, double dSy, double& r2, double& d2)
+{
+    NodeRes x;
+    loadRes(a, n, x);
+    residualFrom(a, x, dSx, dSy, r2, d2);
 }
 // The momentum residual in one pass over a uniform mesh without periodic segments: the
 // element phase of k_mevp_fused with alpha = 1 and no previous stress (Pi sigma_VP(eps(u)),
 // as k_vp_stress), its weak divergence into LDS, then every owned non-Dirichlet node forms
 // R from the LDS contributions (in nodeTileUpdate's order) and adds R^2 and
 // (M_L A tau_a)^2 to its thread's compensated sums; one (sum, compensation) pair per field
-// and block for k_sum_final<2>. Replaces k_vp_stress, k_divergence and k_residual_nodes,
-// and the stress and divergence never touch HBM.
+// and block for k_sum_final<2>. Replaces k_strength, k_vp_stress, k_divergence and
+// k_residual_nodes: the Gauss-point strength comes from the element's h and A in the
+// element phase, and the stress and divergence never touch HBM.
 template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_residual_fused(FusedArgs f, double* part)
 {
     using T = FusedTile<CG>;
@@ -1612,8 +1652,11 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_residual_f
             if (ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny) {
                 const int e = ex + a.nx * ey;
                 if (a.ocean[e]) {
+                    double pk[MomLayout<CG>::Q];
+                    gaussStrength<CG, DG>(f.e.h, f.e.A, f.e.ne, e, f.e.Pstar, pk);
                     Acc<double> acc1[S], acc2[S], acc3[S];
-                    elementAffineCompute<CG, DG, double, double, true>(f.e, e, acc1, acc2, acc3);
+                    elementAffineCompute<CG, DG, double, double, true>(
+                        f.e, e, acc1, acc2, acc3, [&](int q) -> double { return pk[q]; });
                     double s11[S], s12[S], s22[S];
 #pragma unroll
                     for (int k = 0; k < S; ++k) {
@@ -1635,26 +1678,45 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_residual_f
         const int nx0 = CG * ex0, ny0 = CG * ey0;
         const int W = (ex0 + T::TX >= a.nx) ? row - nx0 : CG * T::TX;
         const int H = (ey0 + T::TY >= a.ny) ? CG * a.ny + 1 - ny0 : CG * T::TY;
-        for (int idx = threadIdx.x; idx < W * H; idx += kBlock) {
-            const int nix = nx0 + idx % W, niy = ny0 + idx / W;
-            const int n = nix + row * niy;
-            if (a.dirNode[n])
-                continue;
-            double dx = 0.0, dy = 0.0;
-            forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
-                const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
-                dx -= contrib[le][j][0];
-                dy -= contrib[le][j][1];
-            });
-            double r2 = 0.0, d2 = 0.0;
-            residualNode(a, n, dx, dy, r2, d2);
-            double t, err;
-            twoSum(sum[0], r2, t, err);
-            sum[0] = t;
-            comp[0] += err;
-            twoSum(sum[1], d2, t, err);
-            sum[1] = t;
-            comp[1] += err;
+        // NBB nodes per thread at a time, all their values loaded before any is used
+        // (Dirichlet nodes load theirs too and skip the sum); a thread still adds its
+        // nodes in increasing order
+        constexpr int NBB = 2;
+        for (int base = threadIdx.x; base < W * H; base += NBB * kBlock) {
+            NodeRes x[NBB];
+            bool use[NBB];
+#pragma unroll
+            for (int bb = 0; bb < NBB; ++bb) {
+                const int idx = base + bb * kBlock;
+                use[bb] = idx < W * H;
+                if (use[bb]) {
+                    const int n = (nx0 + idx % W) + row * (ny0 + idx / W);
+                    use[bb] = a.dirNode[n] == 0;
+                    loadRes(a, n, x[bb]);
+                }
+            }
+#pragma unroll
+            for (int bb = 0; bb < NBB; ++bb) {
+                if (!use[bb])
+                    continue;
+                const int idx = base + bb * kBlock;
+                const int nix = nx0 + idx % W, niy = ny0 + idx / W;
+                double dx = 0.0, dy = 0.0;
+                forEachAdjacentXY<CG, true>(nix, niy, a.nx, a.ny, [&](int ex, int ey, int j) {
+                    const int le = (ex - ex0 + 1) + T::HX * (ey - ey0 + 1);
+                    dx -= contrib[le][j][0];
+                    dy -= contrib[le][j][1];
+                });
+                double r2 = 0.0, d2 = 0.0;
+                residualFrom(a, x[bb], dx, dy, r2, d2);
+                double t, err;
+                twoSum(sum[0], r2, t, err);
+                sum[0] = t;
+                comp[0] += err;
+                twoSum(sum[1], d2, t, err);
+                sum[1] = t;
+                comp[1] += err;
+            }
         }
     }
     blockTwoSum<2>(red, sum, comp);
@@ -4211,9 +4273,6 @@ template <int CG, int DG> struct DynImpl final : DynBase {
         require(deltaT > 0, "begin_iteration (or update) must set dt first", NSDG_ERR_STATE);
         if (fusedSubcycle() && mesh.uniform) {
             Timed t(prof, s, "residual");
-            k_strength<CG, DG, double><<<blocksFor(ne), kBlock, 0, s>>>(static_cast<int>(ne), h.p, A.p, p.Pstar,
-                strength.p);
-            checkLaunch("strength");
             FusedArgs f {};
             f.e = elemArgs();
             f.e.invAlpha = 1.0;
