@@ -1454,7 +1454,8 @@ struct MevpNode {
 // local element le, its node j, component c; two-colour stripe order, land contributions
 // exact zeros) and take the FP64 momentum update of the policy P; u, v are read from
 // a.u, a.v and written to uOut, vOut. Each thread updates up to NB
-// nodes of one column, all their loads issued first.
+// nodes of one column, all their loads issued first (Dirichlet nodes load theirs too, so
+// that no load waits for the Dirichlet flag).
 template <int CG, class T, class P = MevpNode, class G>
 __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const G& contrib, int ex0, int ey0, double* uOut,
     double* vOut, const P& pol = P {})
@@ -1477,7 +1478,7 @@ __device__ __forceinline__ void nodeTileUpdate(const NodeArgs& a, const G& contr
                 live[b] = r < H;
                 const int n = nix + row * (ny0 + r);
                 dir[b] = live[b] ? a.dirNode[n] != 0 : true;
-                if (live[b] && (P::kAllNodes || !dir[b]))
+                if (live[b])
                     pol.load(a, n, nix, ny0 + r, in[b]);
             }
 #pragma unroll
@@ -1845,7 +1846,8 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
     }
     __syncthreads();
     // sub-step 1, the nodes of the inner elements (within the domain) into LDS, in batches
-    // of NBB nodes per thread (each batch's frozen terms loaded before its gathers)
+    // of NBB nodes per thread (each batch's frozen terms loaded before its gathers, for
+    // Dirichlet nodes too, so that the loads do not wait for the Dirichlet flag)
     constexpr int NBB = 2;
 #pragma unroll
     for (int b0 = 0; b0 < NB; b0 += NBB) {
@@ -1858,7 +1860,7 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
             live[bb] = b0 + bb < NB && i < NWH && nix >= 0 && nix <= CG * a.nx && niy >= 0 && niy <= CG * a.ny;
             const int n = nix + row * niy;
             dir[bb] = live[bb] ? a.dirNode[n] != 0 : true;
-            if (live[bb] && !dir[bb])
+            if (live[bb])
                 loadFrozenAt(a, static_cast<uint32_t>(n), in[bb]);
         }
 #pragma unroll
