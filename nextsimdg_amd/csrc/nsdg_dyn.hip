@@ -1787,7 +1787,8 @@ struct MevpNodeWindow {
 // indices iter and iter + 1, as two single passes round it; the divergence and sub-step 2
 // use the rounded values; the node update stays FP64).
 template <int CG, int DG, class R>
-__global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArgs f)
+__global__ __launch_bounds__((PairTile<CG, R>::BLOCK))
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 1, sizeof(R) == 8 ? 3 : 8))) void k_mevp_pair(FusedArgs f)
 {
     using T = PairTile<CG, R>;
     constexpr int N = T::N, S = cg2dgstress(CG), K = CG + 1;
@@ -1810,7 +1811,9 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
     const bool inDomain = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
     const int e = inDomain ? ex + a.nx * ey : 0;
     R s11[S], s12[S], s22[S]; // sigma^(k+1) of this thread's element
-    // the element's Gauss-point strength, read once for both sub-steps
+    // the element's Gauss-point strength for sub-step 1. Sub-step 2 reads it again in FP64
+    // (from the L2): the 18 VGPRs it frees give the FP64 pass a third block per CU
+    // (occupancy 3, 168 VGPRs; LDS 3 x 50 KB). FP32 keeps it in registers.
     constexpr int Q = MomLayout<CG>::Q;
     R pk[Q];
     if (inDomain) {
@@ -1821,6 +1824,12 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
             pk[q] = ldAt(Pg + q * f.e.ne, eoff);
     }
     const auto kept = [&](int q) -> R { return pk[q]; };
+    const auto strength2 = [&](int q) -> R {
+        if constexpr (sizeof(R) == 8)
+            return ldAt(strengthOf<R>(f.e) + q * f.e.ne, static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R)));
+        else
+            return pk[q];
+    };
     {
         R tx[N], ty[N];
 #pragma unroll
@@ -1918,7 +1927,7 @@ __global__ __launch_bounds__((PairTile<CG, R>::BLOCK)) void k_mevp_pair(FusedArg
                             lv[c + K * r] = vS[bx + c + T::NW * (by + r)];
                         }
                 },
-                [&](int c, int k) -> R { return sigS[(c * S + k) * NI + le2]; }, kept);
+                [&](int c, int k) -> R { return sigS[(c * S + k) * NI + le2]; }, strength2);
             R n11[S], n12[S], n22[S];
 #pragma unroll
             for (int k = 0; k < S; ++k) {
