@@ -1788,7 +1788,7 @@ struct MevpNodeWindow {
 // use the rounded values; the node update stays FP64).
 template <int CG, int DG, class R>
 __global__ __launch_bounds__((PairTile<CG, R>::BLOCK))
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 1, sizeof(R) == 8 ? 3 : 8))) void k_mevp_pair(FusedArgs f)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 1))) void k_mevp_pair(FusedArgs f)
 {
     using T = PairTile<CG, R>;
     constexpr int N = T::N, S = cg2dgstress(CG), K = CG + 1;
