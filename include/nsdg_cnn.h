@@ -4,7 +4,8 @@
  * velocity and stress for a fixed subcycle count (VPCGDynamicsKernel.hpp:63-94). This
  * interface adds a small convolutional network on the element grid that predicts the
  * converged velocity of a timestep from the state and forcing, trained on the GPU from
- * pairs the GPU solver generates, and a residual-gated solve that starts from it.
+ * pairs the GPU solver generates, and a residual-gated solve that starts from it. The
+ * gated start is opt-in: nsdg_dyn_update never calls it.
  *
  * Network: 4 layers of 3x3 convolutions ('same' zero padding), 32 channels each, ReLU
  * between layers, linear output; inputs and activations BF16 in NHWC layout
