@@ -398,8 +398,9 @@ template <int DG, bool UNI, bool FINISH> __global__ __launch_bounds__(kBlock) vo
                 }
             } else {
                 using L = TrGeo<DG>;
-                const double X1 = a.ops[L::X1 * ne + e], X2 = a.ops[L::X2 * ne + e], X3 = a.ops[L::X3 * ne + e];
-                const double Y1 = a.ops[L::Y1 * ne + e], Y2 = a.ops[L::Y2 * ne + e], Y3 = a.ops[L::Y3 * ne + e];
+                const size_t sne = static_cast<size_t>(ne);
+                const double X1 = a.ops[L::X1 * sne + e], X2 = a.ops[L::X2 * sne + e], X3 = a.ops[L::X3 * sne + e];
+                const double Y1 = a.ops[L::Y1 * sne + e], Y2 = a.ops[L::Y2 * sne + e], Y3 = a.ops[L::Y3 * sne + e];
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     const double xi = gauss_point(G, q % G), eta = gauss_point(G, q / G);
