@@ -1821,12 +1821,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 1))) void k_mevp_pair(Fu
         const uint32_t eoff = static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R));
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            pk[q] = ldAt(Pg + static_cast<size_t>(q) * f.e.ne, eoff);
+            pk[q] = ldAt(Pg + static_cast<uint32_t>(q) * static_cast<uint32_t>(f.e.ne), eoff); // < 2^32: ne < 2^29
     }
     const auto kept = [&](int q) -> R { return pk[q]; };
     const auto strength2 = [&](int q) -> R {
         if constexpr (sizeof(R) == 8)
-            return ldAt(strengthOf<R>(f.e) + static_cast<size_t>(q) * f.e.ne,
+            return ldAt(strengthOf<R>(f.e) + static_cast<uint32_t>(q) * static_cast<uint32_t>(f.e.ne),
                 static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(R)));
         else
             return pk[q];
