@@ -1742,7 +1742,7 @@ template <int CG, int DG> __global__ __launch_bounds__(kBlock) void k_residual_f
 // k_mevp_fused, in the same order, from the same operands (a halo value identically in
 // every tile that computes it); the compiler may still fuse a multiply-add differently
 // in the two kernels, which moves a value by an ulp.
-// FP64: 13 x 13 owned elements on 256 threads (its LDS allows two blocks per CU). FP32,
+// FP64: 13 x 13 owned elements on 256 threads (51 KB of LDS: three blocks per CU). FP32,
 // bound by instruction issue and with half the LDS per value: 29 x 13 on 512 threads, 2.47
 // element updates per owned element and two subcycles instead of 2.67.
 template <int CG, class R> struct PairTile {
