@@ -226,26 +226,31 @@ def _traffic(path, key):
 def fp32_variant(nx, steps, warmup, err_steps, seed=0):
     """The FP32 variant on heterogeneous ice (h = 0.3 + LogNormal(0, 0.3), A ~ Beta(8, 1)):
     throughput next to the FP64 path on the same inputs, and the per-field relative L2
-    error of FP32 against FP64 after err_steps model steps, beside the conditioning floor:
-    the same FP64 run with h perturbed by a relative 1e-7 (one FP32 rounding)."""
+    error of FP32 against FP64 after 1, 10 and err_steps model steps, each beside the
+    conditioning floor: the same FP64 run with h perturbed by a relative 1e-7 (one FP32
+    rounding)."""
     import numpy as np
 
     kind = "heterogeneous"
     t64, _ = measure(nx, nx, steps, warmup, seed=seed, prec=0, kind=kind)
     t32, st32 = measure(nx, nx, steps, warmup, seed=seed, prec=1, kind=kind)
+    horizons = sorted({h for h in (1, 10, err_steps) if 1 <= h <= err_steps})
     fields = {}
     for prec, pert in ((0, 0.0), (1, 0.0), ("floor", 1e-7)):
         k, off, L = make_kernel(nx, nx, seed, 0 if prec == "floor" else prec, kind, pert)
-        run_steps(k, off, L, 0, err_steps, 120.0)
-        k.synchronize()
-        f = {n: k.get_cg(n) for n in ("u", "v")}
-        f.update({n: k.getDGData(n, 6) for n in ("hice", "cice")})
-        f.update({n: k.getDGData(n, 8) for n in ("s11", "s12", "s22")})
-        fields[prec] = f
+        done = 0
+        for h in horizons:
+            run_steps(k, off, L, done, h - done, 120.0)
+            done = h
+            k.synchronize()
+            f = {n: k.get_cg(n) for n in ("u", "v")}
+            f.update({n: k.getDGData(n, 6) for n in ("hice", "cice")})
+            f.update({n: k.getDGData(n, 8) for n in ("s11", "s12", "s22")})
+            fields[(prec, h)] = f
         del k
-    def rel(a):
-        return {n: float(f"{np.linalg.norm(fields[a][n] - fields[0][n]) / np.linalg.norm(fields[0][n]):.3e}")
-                for n in fields[0]}
+    def rel(a, h=err_steps):
+        ref = fields[(0, h)]
+        return {n: float(f"{np.linalg.norm(fields[(a, h)][n] - ref[n]) / np.linalg.norm(ref[n]):.3e}") for n in ref}
     iters = nx * nx * 100 * steps
     return {
         "workload": f"mEVP dynamics step, {nx}x{nx} CG2/DG2, heterogeneous ice, FP32 stress/strength storage "
@@ -258,6 +263,7 @@ def fp32_variant(nx, steps, warmup, err_steps, seed=0):
         "rel_l2_vs_fp64": rel(1),
         "fp64_rel_l2_under_1e-7_input_perturbation": rel("floor"),
         "error_after_steps": err_steps,
+        "rel_l2_by_steps": {str(h): {"fp32_vs_fp64": rel(1, h), "fp64_floor": rel("floor", h)} for h in horizons},
     }
 
 
