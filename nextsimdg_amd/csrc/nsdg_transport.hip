@@ -130,24 +130,33 @@ template <int DG> __global__ void k_normal_velocity(MeshView m, const double* vx
         const int n0 = iy * (nx + 1) + ix;
         const double t0 = edgeDx(m, m.vxy[2 * (n0 + 1)] - m.vxy[2 * n0]);
         const double t1 = m.vxy[2 * (n0 + 1) + 1] - m.vxy[2 * n0 + 1];
-        if (iy > 0 && m.ocean[t - nx]) {
-            const int c = t - nx;
+        // both cells' coefficients are loaded before their ocean flags are tested
+        double dx[DG], dy[DG];
+        const bool hb = iy > 0, ht = iy < ny;
+        if (hb)
 #pragma unroll
             for (int i = 0; i < DG; ++i) {
-                cx[i] = vx[static_cast<size_t>(i) * ne + c];
-                cy[i] = vy[static_cast<size_t>(i) * ne + c];
+                cx[i] = vx[static_cast<size_t>(i) * ne + (t - nx)];
+                cy[i] = vy[static_cast<size_t>(i) * ne + (t - nx)];
             }
+        if (ht)
+#pragma unroll
+            for (int i = 0; i < DG; ++i) {
+                dx[i] = vx[static_cast<size_t>(i) * ne + t];
+                dy[i] = vy[static_cast<size_t>(i) * ne + t];
+            }
+        if (hb && m.ocean[t - nx]) {
             trace<DG, 2>(cx, tx);
             trace<DG, 2>(cy, ty);
 #pragma unroll
             for (int k = 0; k < E; ++k)
                 acc[k] += 0.5 * (-t1 * tx[k] + t0 * ty[k]);
         }
-        if (iy < ny && m.ocean[t]) {
+        if (ht && m.ocean[t]) {
 #pragma unroll
             for (int i = 0; i < DG; ++i) {
-                cx[i] = vx[static_cast<size_t>(i) * ne + t];
-                cy[i] = vy[static_cast<size_t>(i) * ne + t];
+                cx[i] = dx[i];
+                cy[i] = dy[i];
             }
             trace<DG, 0>(cx, tx);
             trace<DG, 0>(cy, ty);
@@ -166,23 +175,32 @@ template <int DG> __global__ void k_normal_velocity(MeshView m, const double* vx
         const double t0 = edgeDx(m, m.vxy[2 * (n0 + nx + 1)] - m.vxy[2 * n0]);
         const double t1 = m.vxy[2 * (n0 + nx + 1) + 1] - m.vxy[2 * n0 + 1];
         const int cl = ix - 1 + nx * iy, cr = ix + nx * iy;
-        if (ix > 0 && m.ocean[cl]) {
+        double dx[DG], dy[DG];
+        const bool hl = ix > 0, hr = ix < nx;
+        if (hl)
 #pragma unroll
             for (int i = 0; i < DG; ++i) {
                 cx[i] = vx[static_cast<size_t>(i) * ne + cl];
                 cy[i] = vy[static_cast<size_t>(i) * ne + cl];
             }
+        if (hr)
+#pragma unroll
+            for (int i = 0; i < DG; ++i) {
+                dx[i] = vx[static_cast<size_t>(i) * ne + cr];
+                dy[i] = vy[static_cast<size_t>(i) * ne + cr];
+            }
+        if (hl && m.ocean[cl]) {
             trace<DG, 1>(cx, tx);
             trace<DG, 1>(cy, ty);
 #pragma unroll
             for (int k = 0; k < E; ++k)
                 acc[k] += 0.5 * (t1 * tx[k] - t0 * ty[k]);
         }
-        if (ix < nx && m.ocean[cr]) {
+        if (hr && m.ocean[cr]) {
 #pragma unroll
             for (int i = 0; i < DG; ++i) {
-                cx[i] = vx[static_cast<size_t>(i) * ne + cr];
-                cy[i] = vy[static_cast<size_t>(i) * ne + cr];
+                cx[i] = dx[i];
+                cy[i] = dy[i];
             }
             trace<DG, 3>(cx, tx);
             trace<DG, 3>(cy, ty);
