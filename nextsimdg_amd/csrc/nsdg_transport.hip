@@ -565,6 +565,13 @@ __global__ void k_limit(MeshView m, double* f, int ne, bool doMax, double maxv, 
             }
         }
     }
+    // a cell the limits leave bit for bit as it was is not written back (most cells)
+    bool moved = false;
+#pragma unroll
+    for (int i = 0; i < DG; ++i)
+        moved = moved || __double_as_longlong(c[i]) != __double_as_longlong(c0[i]);
+    if (!moved)
+        return;
 #pragma unroll
     for (int i = 0; i < DG; ++i)
         f[static_cast<size_t>(i) * ne + e] = c[i];
