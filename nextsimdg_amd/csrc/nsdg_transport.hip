@@ -705,10 +705,26 @@ void deviceFill(double* p, size_t n, double v, hipStream_t s)
     k_fill<<<blocksFor(n), kBlock, 0, s>>>(p, n, v);
     checkLaunch("fill");
 }
+// Device-to-device copy as a plain kernel of 16-byte accesses (the runtime's blit kernel
+// moves these field-sized copies at about half the rate).
+__global__ void k_copy(double* dst, const double* src, size_t n)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (2 * t + 1 < n)
+        reinterpret_cast<double2*>(dst)[t] = reinterpret_cast<const double2*>(src)[t];
+    else if (2 * t < n)
+        dst[2 * t] = src[2 * t];
+}
 void deviceCopy(double* dst, const double* src, size_t n, hipStream_t s)
 {
-    if (n)
+    if (!n)
+        return;
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u) != 0) {
         NSDG_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    k_copy<<<blocksFor((n + 1) / 2), kBlock, 0, s>>>(dst, src, n);
+    checkLaunch("copy");
 }
 
 void CG2DGOps::build(int cg_, int dg_, const DeviceMesh& mesh, hipStream_t s)
